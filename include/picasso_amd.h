@@ -1,0 +1,221 @@
+/* picasso_amd.h -- C ABI of the MI355X (gfx950) particle<->grid transfer
+ * library.
+ *
+ * The reference (Picasso) exposes this path as header-only C++ device
+ * functions called once per particle inside the caller's Kokkos kernel; there
+ * is no FFI.  Per-particle calls cannot cross a shared-library boundary
+ * efficiently, so each entry point here is the batch form of one reference
+ * function: it runs that function for particles [0, np) on the context's HIP
+ * stream.  Argument meaning and order follow the reference; the citations
+ * name the function each entry point replaces.
+ *
+ * Conventions
+ *  - Every call returns int: 0 (PIC_OK) or a negative pic_status.  Nothing
+ *    throws across the ABI.  Calls are asynchronous on the context's stream;
+ *    errors detected on the device (a stencil leaving its grid window, a
+ *    particle outside the binned cells) are latched in the context and
+ *    returned by pic_synchronize().
+ *  - All buffers are caller-owned device memory.  Particle fields are
+ *    described by pic_pfield (SoA, AoS and Cabana AoSoA all map onto it),
+ *    grid fields by pic_grid (any i/j/k/component strides).
+ *  - p2g accumulates (+=) like Kokkos::ScatterView::access() in the
+ *    reference (Picasso_APIC.hpp:69-73); PIC_P2G_ZERO_FIRST zeroes the grid
+ *    windows first, as GridOperator does before a scatter
+ *    (Picasso_GridOperator.hpp:532-533).
+ *  - g2p overwrites the particle field (Picasso_APIC.hpp:463-471,
+ *    Picasso_PolyPIC.hpp:260), except a staggered PolyPIC g2p, which only
+ *    rewrites the column it works on (Picasso_PolyPIC.hpp:328).
+ *  - dtype PIC_F64: every field is double.  PIC_F32: every field is float;
+ *    the position->cell mapping runs in FP64, p2g through bins accumulates
+ *    with compensated (two-sum) FP32 sums.
+ */
+#ifndef PICASSO_AMD_H
+#define PICASSO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIC_API_VERSION 1
+
+typedef enum pic_status {
+    PIC_OK = 0,
+    PIC_ERR_ARG = -1,         /* invalid argument (null, bad enum, bad size) */
+    PIC_ERR_BOUNDS = -2,      /* a stencil or cell fell outside its window */
+    PIC_ERR_HIP = -3,         /* a HIP runtime call failed */
+    PIC_ERR_ALLOC = -4,       /* device allocation failed */
+    PIC_ERR_UNSUPPORTED = -5, /* combination not implemented */
+    PIC_ERR_CAPACITY = -6     /* a size exceeds a documented maximum */
+} pic_status;
+
+/* Grid entity kinds (Picasso_FieldTypes.hpp:125-168). */
+typedef enum pic_entity {
+    PIC_NODE = 0,
+    PIC_CELL = 1,
+    PIC_FACE_I = 2,
+    PIC_FACE_J = 3,
+    PIC_FACE_K = 4,
+    PIC_EDGE_I = 5,
+    PIC_EDGE_J = 6,
+    PIC_EDGE_K = 7
+} pic_entity;
+
+typedef enum pic_dtype { PIC_F64 = 0, PIC_F32 = 1 } pic_dtype;
+
+enum { PIC_P2G_ZERO_FIRST = 1u };
+
+/* Uniform local mesh (Picasso_UniformMesh.hpp:84-190): `low` is the physical
+ * position of local node (0,0,0), ghost layers included; `h` the isotropic
+ * cell size.  Entity lattices sit at low + h/2 in their cell-centred
+ * dimensions. */
+typedef struct pic_mesh {
+    double low[3];
+    double h;
+} pic_mesh;
+
+/* A window of a grid field: local indices [lo, lo+n) in each dimension.
+ * Element (i,j,k,c) lives at
+ *   data[(i-lo[0])*stride[0] + (j-lo[1])*stride[1] + (k-lo[2])*stride[2]
+ *        + c*cstride]
+ * (elements of the call's dtype).  Stencils must stay inside the window. */
+typedef struct pic_grid {
+    void* data;
+    int32_t lo[3];
+    int32_t n[3];
+    int64_t stride[3];
+    int64_t cstride;
+} pic_grid;
+
+/* A particle field.  Element (p, c) lives at
+ *   data[(p >> vlen_log2) * tile_stride + (p & ((1 << vlen_log2) - 1))
+ *        + c * cstride]
+ * SoA: vlen_log2 = 0, tile_stride = 1, cstride = np.  AoS with q components:
+ * vlen_log2 = 0, tile_stride = q, cstride = 1.  Cabana AoSoA with vector
+ * length V and a struct of q scalars (Picasso_ParticleList.hpp:90-138):
+ * vlen_log2 = log2 V, tile_stride = V*q, cstride = V (component offsets
+ * folded into data).  Matrix fields are row-major: APIC c_p component
+ * r*3 + d is c_p(r, d); PolyPIC modes component r*ncomp + d. */
+typedef struct pic_pfield {
+    void* data;
+    int32_t vlen_log2;
+    int64_t tile_stride;
+    int64_t cstride;
+} pic_pfield;
+
+/* Particles binned by pic_bin_sort: perm[s] is the particle in sorted slot
+ * s; brick b holds slots [bin_start[b], bin_start[b+1]).  Cells (node
+ * lattice, floor((x-low)/h)) [cell_lo, cell_lo+ncell) are grouped in bricks
+ * of brick[3] cells. */
+typedef struct pic_bins {
+    const uint32_t* perm;
+    const uint32_t* bin_start;
+    int32_t cell_lo[3];
+    int32_t ncell[3];
+    int32_t brick[3];
+} pic_bins;
+
+typedef struct pic_context pic_context;
+
+/* Context: device, stream, device-side error word, scratch memory. */
+int pic_context_create(pic_context** ctx, int device);
+int pic_context_destroy(pic_context* ctx);
+/* hipStream_t passed as void*; NULL selects the null stream. */
+int pic_set_stream(pic_context* ctx, void* stream);
+/* Wait for the stream; return the first latched device error and clear it. */
+int pic_synchronize(pic_context* ctx);
+const char* pic_status_string(int status);
+int pic_api_version(void);
+
+/* APIC::p2g (Picasso_APIC.hpp:55-140 orders 2/3 Node/Cell; :148-232 orders
+ * 2/3 Face/Edge; :240-319 order 1 Node/Cell; :327-402 order 1 Face/Edge).
+ * Reference signature p2g(m_p, c_p, m_i, mu_i, sd): x and the mesh stand in
+ * for the spline data sd (createSpline, Picasso_ParticleInterpolation.hpp:
+ * 88-105).  c_p has 12 components (4x3).  mu_i has 3 components for
+ * Node/Cell, 1 for Face/Edge.  bins may be NULL. */
+int pic_apic_p2g(pic_context* ctx, int order, int entity, int dtype,
+                 const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                 const pic_pfield* m_p, const pic_pfield* c_p,
+                 const pic_grid* m_i, const pic_grid* mu_i,
+                 const pic_bins* bins, uint32_t flags);
+
+/* APIC::g2p (Picasso_APIC.hpp:407-472 Node/Cell; :477-544 Face/Edge).
+ * Reference signature g2p(u_i, c_p, sd).  Overwrites all 12 components. */
+int pic_apic_g2p(pic_context* ctx, int order, int entity, int dtype,
+                 const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                 const pic_grid* u_i, const pic_pfield* c_p,
+                 const pic_bins* bins);
+
+/* Number of PolyPIC modes per component for a spline order: 8 (linear, the
+ * reference's), 27 (orders 2 and 3, the {1,y,y^2}^3 basis). */
+int pic_polypic_nmode(int order);
+
+/* PolyPIC::p2g (Picasso_PolyPIC.hpp:39-130 Node/Cell; :138-229 Face/Edge).
+ * Reference signature p2g(m_p, u_p, c_p, mu_i, m_i, dt, sd) -- field before
+ * mass.  u_p has nmode*3 components, c_p nmode*ncomp; mu_i has ncomp
+ * components for Node/Cell and 1 for Face/Edge (component entity dim when
+ * ncomp == 3, else 0). */
+int pic_polypic_p2g(pic_context* ctx, int order, int entity, int dtype,
+                    const pic_mesh* mesh, int64_t np, int ncomp,
+                    const pic_pfield* x, const pic_pfield* m_p,
+                    const pic_pfield* u_p, const pic_pfield* c_p,
+                    const pic_grid* mu_i, const pic_grid* m_i, double dt,
+                    const pic_bins* bins, uint32_t flags);
+
+/* PolyPIC::g2p (Picasso_PolyPIC.hpp:235-293 Node/Cell; :299-359 Face/Edge). */
+int pic_polypic_g2p(pic_context* ctx, int order, int entity, int dtype,
+                    const pic_mesh* mesh, int64_t np, int ncomp,
+                    const pic_pfield* x, const pic_grid* u_i,
+                    const pic_pfield* c_p, const pic_bins* bins);
+
+/* Grid velocity u_i = mu_i / m_i over the window of m_i (no reference
+ * function; doc/basic/explict_nearly_incompressible_flow.md:18-23 without
+ * forces).  Nodes with m_i == 0 get u_i = 0.  For PIC_F32, nodes with
+ * m_i < rel_mass_eps * max(m_i) divide in FP64. */
+int pic_grid_normalize(pic_context* ctx, int dtype, int ncomp,
+                       const pic_grid* m_i, const pic_grid* mu_i,
+                       const pic_grid* u_i, double rel_mass_eps);
+
+/* Velocity gradient at the particles, gradu(i,j) = du_i/dx_j (component
+ * i*3+j), from a 3-component Node/Cell grid field (G2P::gradient,
+ * Picasso_ParticleInterpolation.hpp:165-181). */
+int pic_g2p_gradu(pic_context* ctx, int order, int entity, int dtype,
+                  const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                  const pic_grid* u_i, const pic_pfield* gradu,
+                  const pic_bins* bins);
+
+/* F_p <- (I + dt*gradu) F_p, det(F_p) and F_p^{-1} in FP64
+ * (Picasso_BatchedLinearAlgebra.hpp:3806-3834 product, :4335-4350 det,
+ * :4624-4685 inverse).  gradu is of gradu_dtype; F, det, Finv are double. */
+int pic_update_F(pic_context* ctx, int64_t np, int gradu_dtype,
+                 const pic_pfield* gradu, double dt, const pic_pfield* F,
+                 const pic_pfield* det, const pic_pfield* Finv);
+
+/* Number of bricks for a binned region (the bin_start array needs one more
+ * entry); returns a negative pic_status on invalid sizes. */
+int64_t pic_bin_count(const int32_t ncell[3], const int32_t brick[3]);
+
+/* Stable counting sort of particles by brick-major cell key (no reference
+ * function; Picasso has no particle sort).  key = brick * brick_volume +
+ * cell_in_brick, both i-major.  Writes keys[np], perm[np] and
+ * bin_start[nbricks+1]; equal keys keep their input order. */
+int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh,
+                 int64_t np, const pic_pfield* x, const int32_t cell_lo[3],
+                 const int32_t ncell[3], const int32_t brick[3],
+                 uint32_t* keys, uint32_t* perm, uint32_t* bin_start);
+
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on
+ * the context's stream and pic_kernel_time() returns the accumulated
+ * milliseconds and launch count of the named kernel family
+ * ("apic_p2g", "apic_g2p", "normalize", "bin_sort", ...). */
+int pic_set_timing(pic_context* ctx, int enable);
+int pic_kernel_time(pic_context* ctx, const char* name, double* ms,
+                    int64_t* launches);
+int pic_reset_timing(pic_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,303 @@
+"""MI355X (gfx950) particle<->grid transfers with Picasso's operator interface.
+
+The operators mirror the reference's call signatures at batch granularity:
+
+    sd = create_spline(FieldLocation.Node, order, mesh, x)     # createSpline
+    APIC.p2g(ctx, m_p, c_p, m_i, mu_i, sd)                      # APIC::p2g
+    APIC.g2p(ctx, u_i, c_p, sd)                                 # APIC::g2p
+    PolyPIC.p2g(ctx, m_p, u_p, c_p, mu_i, m_i, dt, sd)          # PolyPIC::p2g
+    PolyPIC.g2p(ctx, u_i, c_p, sd)                              # PolyPIC::g2p
+
+(Picasso_APIC.hpp:55-544, Picasso_PolyPIC.hpp:39-359,
+Picasso_ParticleInterpolation.hpp:88-105).  Tensors are torch CUDA (HIP)
+tensors used only as device memory; every operator runs the HIP kernels of
+libpicasso_amd.so through its C ABI on torch's current stream.
+
+Particle tensors are 2-D [np, ncomp] with any strides (a transposed
+[ncomp, np] tensor is SoA).  Grid tensors are 3-D [n0, n1, n2] (scalar) or
+4-D [n0, n1, n2, ncomp], any strides, covering local indices
+[lo, lo + n) of the mesh's entity lattice.
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+from . import _abi
+from ._abi import (CELL, EDGE_I, EDGE_J, EDGE_K, F32, F64, FACE_I, FACE_J, FACE_K, NODE,
+                   PicError, check, lib)
+
+__all__ = ["Context", "Mesh", "GridField", "create_spline", "APIC", "PolyPIC", "FieldLocation",
+           "grid_normalize", "g2p_gradu", "update_F", "bin_sort", "BinSort", "PicError",
+           "lib", "LIB_PATH"]
+
+LIB_PATH = _abi.LIB_PATH
+
+
+class FieldLocation:
+    """Entity kinds (Picasso_FieldTypes.hpp:125-168)."""
+    Node = NODE
+    Cell = CELL
+
+    @staticmethod
+    def Face(dim):
+        return FACE_I + dim
+
+    @staticmethod
+    def Edge(dim):
+        return EDGE_I + dim
+
+
+def _dtype_code(t):
+    import torch
+    if t.dtype == torch.float64:
+        return F64
+    if t.dtype == torch.float32:
+        return F32
+    raise TypeError(f"unsupported dtype {t.dtype}")
+
+
+def _require_device(t, name):
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a GPU tensor")
+
+
+class Context:
+    """Owns a pic_context bound to a device; launches on torch's current
+    stream of that device (re-read on every call)."""
+
+    def __init__(self, device=0):
+        self.device = device
+        h = ctypes.c_void_p()
+        check(lib.pic_context_create(ctypes.byref(h), device), "pic_context_create")
+        self.handle = h
+
+    def _bind_stream(self):
+        import torch
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        check(lib.pic_set_stream(self.handle, ctypes.c_void_p(s)), "pic_set_stream")
+        return self.handle
+
+    def synchronize(self):
+        check(lib.pic_synchronize(self.handle), "pic_synchronize")
+
+    def set_timing(self, on):
+        check(lib.pic_set_timing(self.handle, int(bool(on))), "pic_set_timing")
+
+    def reset_timing(self):
+        check(lib.pic_reset_timing(self.handle), "pic_reset_timing")
+
+    def kernel_time(self, name):
+        ms = ctypes.c_double()
+        n = ctypes.c_int64()
+        check(lib.pic_kernel_time(self.handle, name.encode(), ctypes.byref(ms), ctypes.byref(n)),
+              "pic_kernel_time")
+        return ms.value, n.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.pic_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class Mesh:
+    """Uniform local mesh: low = physical position of local node (0,0,0)
+    (ghosts included), h = cell size (Picasso_UniformMesh.hpp:84-190)."""
+    low: Sequence[float]
+    h: float
+
+    def c(self):
+        return _abi.Mesh((ctypes.c_double * 3)(*[float(v) for v in self.low]), float(self.h))
+
+
+@dataclass
+class GridField:
+    """A grid window: tensor [n0,n1,n2(,ncomp)] for local indices lo..lo+n."""
+    tensor: object
+    lo: Sequence[int] = (0, 0, 0)
+
+    @property
+    def ncomp(self):
+        return 1 if self.tensor.dim() == 3 else self.tensor.shape[3]
+
+    def c(self):
+        t = self.tensor
+        _require_device(t, "grid field")
+        st = t.stride()
+        return _abi.Grid(t.data_ptr(), (ctypes.c_int32 * 3)(*[int(v) for v in self.lo]),
+                         (ctypes.c_int32 * 3)(*t.shape[:3]), (ctypes.c_int64 * 3)(*st[:3]),
+                         st[3] if t.dim() == 4 else 0)
+
+
+def _pfield(t, ncomp, name):
+    _require_device(t, name)
+    if t.dim() == 1:
+        t = t.unsqueeze(1)
+    if t.dim() != 2 or t.shape[1] != ncomp:
+        raise ValueError(f"{name} must be [np, {ncomp}], got {tuple(t.shape)}")
+    st = t.stride()
+    return _abi.PField(t.data_ptr(), 0, st[0], st[1])
+
+
+@dataclass
+class Spline:
+    """Batched stand-in for the reference's SplineData: location, order,
+    mesh and the particle positions it is evaluated at."""
+    location: int
+    order: int
+    mesh: Mesh
+    x: object
+    bins: Optional["BinSort"] = None
+
+
+def create_spline(location, order, mesh, x, bins=None):
+    """createSpline(Location, InterpolationOrder<order>, local_mesh, x, ...)
+    (Picasso_ParticleInterpolation.hpp:88-105).  Weights, gradients and
+    distances are evaluated inside the transfer kernels."""
+    if order not in (1, 2, 3):
+        raise ValueError("order must be 1, 2 or 3")
+    return Spline(location, order, mesh, x, bins)
+
+
+def _bins_ptr(bins):
+    return ctypes.byref(bins.c()) if bins is not None else None
+
+
+class APIC:
+    @staticmethod
+    def p2g(ctx, m_p, c_p, m_i, mu_i, sd, zero_first=False):
+        """APIC::p2g(m_p, c_p, m_i, mu_i, sd) (Picasso_APIC.hpp:55-402).
+        c_p is [np, 12] (row-major 4x3); accumulates into m_i, mu_i."""
+        x = sd.x
+        np_ = x.shape[0]
+        mesh = sd.mesh.c()
+        rc = lib.pic_apic_p2g(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
+                              ctypes.byref(mesh), np_, ctypes.byref(_pfield(x, 3, "x")),
+                              ctypes.byref(_pfield(m_p, 1, "m_p")),
+                              ctypes.byref(_pfield(c_p, 12, "c_p")), ctypes.byref(m_i.c()),
+                              ctypes.byref(mu_i.c()), _bins_ptr(sd.bins),
+                              _abi.P2G_ZERO_FIRST if zero_first else 0)
+        check(rc, "pic_apic_p2g")
+
+    @staticmethod
+    def g2p(ctx, u_i, c_p, sd):
+        """APIC::g2p(u_i, c_p, sd) (Picasso_APIC.hpp:407-544)."""
+        x = sd.x
+        mesh = sd.mesh.c()
+        rc = lib.pic_apic_g2p(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
+                              ctypes.byref(mesh), x.shape[0], ctypes.byref(_pfield(x, 3, "x")),
+                              ctypes.byref(u_i.c()), ctypes.byref(_pfield(c_p, 12, "c_p")),
+                              _bins_ptr(sd.bins))
+        check(rc, "pic_apic_g2p")
+
+
+class PolyPIC:
+    @staticmethod
+    def nmode(order):
+        return check(lib.pic_polypic_nmode(order), "pic_polypic_nmode")
+
+    @staticmethod
+    def p2g(ctx, m_p, u_p, c_p, mu_i, m_i, dt, sd, ncomp=3, zero_first=False):
+        """PolyPIC::p2g(m_p, u_p, c_p, mu_i, m_i, dt, sd)
+        (Picasso_PolyPIC.hpp:39-229).  u_p is [np, nmode*3], c_p
+        [np, nmode*ncomp]."""
+        x = sd.x
+        nm = PolyPIC.nmode(sd.order)
+        mesh = sd.mesh.c()
+        rc = lib.pic_polypic_p2g(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
+                                 ctypes.byref(mesh), x.shape[0], ncomp,
+                                 ctypes.byref(_pfield(x, 3, "x")),
+                                 ctypes.byref(_pfield(m_p, 1, "m_p")),
+                                 ctypes.byref(_pfield(u_p, nm * 3, "u_p")),
+                                 ctypes.byref(_pfield(c_p, nm * ncomp, "c_p")),
+                                 ctypes.byref(mu_i.c()), ctypes.byref(m_i.c()), float(dt),
+                                 _bins_ptr(sd.bins), _abi.P2G_ZERO_FIRST if zero_first else 0)
+        check(rc, "pic_polypic_p2g")
+
+    @staticmethod
+    def g2p(ctx, u_i, c_p, sd, ncomp=3):
+        """PolyPIC::g2p(u_i, c_p, sd) (Picasso_PolyPIC.hpp:235-359)."""
+        x = sd.x
+        nm = PolyPIC.nmode(sd.order)
+        mesh = sd.mesh.c()
+        rc = lib.pic_polypic_g2p(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
+                                 ctypes.byref(mesh), x.shape[0], ncomp,
+                                 ctypes.byref(_pfield(x, 3, "x")), ctypes.byref(u_i.c()),
+                                 ctypes.byref(_pfield(c_p, nm * ncomp, "c_p")),
+                                 _bins_ptr(sd.bins))
+        check(rc, "pic_polypic_g2p")
+
+
+def grid_normalize(ctx, m_i, mu_i, u_i, rel_mass_eps=0.0):
+    """u_i = mu_i / m_i (0 where m_i == 0)."""
+    rc = lib.pic_grid_normalize(ctx._bind_stream(), _dtype_code(m_i.tensor), mu_i.ncomp,
+                                ctypes.byref(m_i.c()), ctypes.byref(mu_i.c()),
+                                ctypes.byref(u_i.c()), float(rel_mass_eps))
+    check(rc, "pic_grid_normalize")
+
+
+def g2p_gradu(ctx, u_i, gradu, sd):
+    """Velocity gradient gradu[p, i*3+j] = du_i/dx_j (G2P::gradient)."""
+    x = sd.x
+    mesh = sd.mesh.c()
+    rc = lib.pic_g2p_gradu(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
+                           ctypes.byref(mesh), x.shape[0], ctypes.byref(_pfield(x, 3, "x")),
+                           ctypes.byref(u_i.c()), ctypes.byref(_pfield(gradu, 9, "gradu")),
+                           _bins_ptr(sd.bins))
+    check(rc, "pic_g2p_gradu")
+
+
+def update_F(ctx, gradu, dt, F, det, Finv):
+    """F <- (I + dt gradu) F, det(F), F^{-1} in FP64."""
+    rc = lib.pic_update_F(ctx._bind_stream(), F.shape[0], _dtype_code(gradu),
+                          ctypes.byref(_pfield(gradu, 9, "gradu")), float(dt),
+                          ctypes.byref(_pfield(F, 9, "F")), ctypes.byref(_pfield(det, 1, "det")),
+                          ctypes.byref(_pfield(Finv, 9, "Finv")))
+    check(rc, "pic_update_F")
+
+
+@dataclass
+class BinSort:
+    """Output of bin_sort: keys (input order), perm (sorted slot ->
+    particle), bin_start (per brick), and the binned region."""
+    keys: object
+    perm: object
+    bin_start: object
+    cell_lo: Sequence[int]
+    ncell: Sequence[int]
+    brick: Sequence[int]
+
+    def c(self):
+        i3 = ctypes.c_int32 * 3
+        return _abi.Bins(self.perm.data_ptr(), self.bin_start.data_ptr(),
+                         i3(*self.cell_lo), i3(*self.ncell), i3(*self.brick))
+
+
+def bin_count(ncell, brick):
+    i3 = ctypes.c_int32 * 3
+    return check(lib.pic_bin_count(i3(*ncell), i3(*brick)), "pic_bin_count")
+
+
+def bin_sort(ctx, mesh, x, cell_lo, ncell, brick, out=None):
+    """Stable counting sort of particles by brick-major cell key."""
+    import torch
+    nb = bin_count(ncell, brick)
+    np_ = x.shape[0]
+    if out is None:
+        keys = torch.empty(np_, dtype=torch.int32, device=x.device)
+        perm = torch.empty(np_, dtype=torch.int32, device=x.device)
+        start = torch.empty(nb + 1, dtype=torch.int32, device=x.device)
+        out = BinSort(keys, perm, start, tuple(cell_lo), tuple(ncell), tuple(brick))
+    i3 = ctypes.c_int32 * 3
+    m = mesh.c()
+    rc = lib.pic_bin_sort(ctx._bind_stream(), _dtype_code(x), ctypes.byref(m), np_,
+                          ctypes.byref(_pfield(x, 3, "x")), i3(*cell_lo), i3(*ncell), i3(*brick),
+                          out.keys.data_ptr(), out.perm.data_ptr(), out.bin_start.data_ptr())
+    check(rc, "pic_bin_sort")
+    return out

@@ -1,0 +1,208 @@
+// APIC particle<->grid transfers (Picasso_APIC.hpp) for gfx950.
+//
+// Two execution schemes share the per-particle arithmetic:
+//  * scatter: one work-item per particle, hardware float atomics into the
+//    grid (any particle order);
+//  * brick: particles binned by pic_bin_sort, one workgroup per brick of
+//    cells, the brick's node footprint accumulated in LDS and flushed with
+//    one global atomic per touched footprint node (pic_apic_brick.hip).
+#include "pic_apic.hpp"
+#include "pic_internal.hpp"
+
+namespace pic {
+
+// ---------------------------------------------------------------------------
+// p2g, scatter scheme.
+template <int ORDER, bool STAG, typename T>
+__global__ __launch_bounds__(256) void k_apic_p2g_scatter(ApicArgs<T> a)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= a.np) return;
+    const int64_t p = a.perm ? int64_t(a.perm[s]) : s;
+
+    ApicParticle<T> pt;
+    load_apic_particle<STAG>(a, p, pt);
+    Spline<ORDER, T> sd;
+    eval_spline<ORDER, T, ORDER == 1, ORDER != 1>(a.lat, pt.x, sd);
+    if (!stencil_inside(sd, a.gmu)) {
+        atomicOr(a.err, ERRBIT_BOUNDS);
+        return;
+    }
+    apic_p2g_particle<ORDER, STAG>(a, pt, sd, [&](int i, int j, int k, const T* mu, T m) {
+        const int64_t off = a.gmu.off(i, j, k);
+        if constexpr (STAG) {
+            atomic_add(a.gmu.data + off, mu[0]);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) atomic_add(a.gmu.data + off + d * a.gmu.cs, mu[d]);
+        }
+        atomic_add(a.gm.data + a.gm.off(i, j, k), m);
+    });
+}
+
+// ---------------------------------------------------------------------------
+// g2p, gather (any particle order).
+template <int ORDER, bool STAG, typename T>
+__global__ __launch_bounds__(256) void k_apic_g2p_gather(ApicArgs<T> a)
+{
+    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (s >= a.np) return;
+    const int64_t p = a.perm ? int64_t(a.perm[s]) : s;
+    const int64_t pb = a.x.base(p);
+    T xp[3] = {a.x.at(pb, 0), a.x.at(pb, 1), a.x.at(pb, 2)};
+    Spline<ORDER, T> sd;
+    eval_spline<ORDER, T, false, true>(a.lat, xp, sd);
+    if (!stencil_inside(sd, a.gmu)) {
+        atomicOr(a.err, ERRBIT_BOUNDS);
+        return;
+    }
+    T cp[12];
+    apic_g2p_particle<ORDER, STAG>(a, sd, [&](int i, int j, int k, T* ue) {
+        const int64_t off = a.gmu.off(i, j, k);
+        if constexpr (STAG) {
+            ue[0] = a.gmu.data[off];
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) ue[d] = a.gmu.data[off + d * a.gmu.cs];
+        }
+    }, cp);
+    const int64_t cb = a.c.base(p);
+#pragma unroll
+    for (int r = 0; r < 12; ++r) a.c.at(cb, r) = cp[r];
+}
+
+// ---------------------------------------------------------------------------
+// Host dispatch.
+
+template <typename T>
+static ApicArgs<T> make_args(pic_context* ctx, int entity, const pic_mesh& mesh,
+                             int64_t np, const pic_pfield* x, const pic_pfield* m,
+                             const pic_pfield* c, const pic_grid* gm, const pic_grid& gmu,
+                             const uint32_t* perm)
+{
+    ApicArgs<T> a;
+    a.lat = make_lattice(mesh, entity);
+    a.np = np;
+    a.perm = perm;
+    a.x = make_pfield<T>(*x);
+    a.m = m ? make_pfield<T>(*m) : PField<T>{};
+    a.c = make_pfield<T>(*c);
+    a.gm = gm ? make_gview<T>(*gm) : GView<T>{};
+    a.gmu = make_gview<T>(gmu);
+    a.sdim = stag_dim(entity) < 0 ? 0 : stag_dim(entity);
+    a.err = ctx->d_err;
+    return a;
+}
+
+template <typename T>
+struct P2GScatterLaunch {
+    ApicArgs<T> a;
+    hipStream_t st;
+    template <int O, bool S, typename U>
+    void operator()()
+    {
+        k_apic_p2g_scatter<O, S, U><<<blocks_for(a.np, 256), 256, 0, st>>>(a);
+    }
+};
+
+template <typename T>
+struct G2PGatherLaunch {
+    ApicArgs<T> a;
+    hipStream_t st;
+    template <int O, bool S, typename U>
+    void operator()()
+    {
+        k_apic_g2p_gather<O, S, U><<<blocks_for(a.np, 256), 256, 0, st>>>(a);
+    }
+};
+
+}  // namespace pic
+
+using namespace pic;
+
+static bool same_window(const pic_grid& a, const pic_grid& b)
+{
+    for (int d = 0; d < 3; ++d)
+        if (a.lo[d] != b.lo[d] || a.n[d] != b.n[d]) return false;
+    return true;
+}
+
+extern "C" int pic_apic_p2g(pic_context* ctx, int order, int entity, int dtype,
+                            const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                            const pic_pfield* m_p, const pic_pfield* c_p,
+                            const pic_grid* m_i, const pic_grid* mu_i,
+                            const pic_bins* bins, uint32_t flags)
+{
+    if (!ctx || !mesh || !(mesh->h > 0.0) || order < 1 || order > 3 ||
+        !valid_entity(entity) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
+        np > int64_t(UINT32_MAX) || !valid_grid(m_i) || !valid_grid(mu_i) ||
+        !same_window(*m_i, *mu_i))
+        return PIC_ERR_ARG;
+    if (np > 0 && (!valid_pfield(x) || !valid_pfield(m_p) || !valid_pfield(c_p)))
+        return PIC_ERR_ARG;
+    if (bins && !bins->perm) return PIC_ERR_ARG;
+    DeviceGuard g(ctx->device);
+    if (!g.ok) return PIC_ERR_HIP;
+    const bool stag = stag_dim(entity) >= 0;
+    if (flags & PIC_P2G_ZERO_FIRST) {
+        int rc = zero_grid(ctx, dtype, *m_i, 1);
+        if (rc) return rc;
+        rc = zero_grid(ctx, dtype, *mu_i, stag ? 1 : 3);
+        if (rc) return rc;
+    }
+    if (np == 0) return PIC_OK;
+    if (bins) {
+        int rc = apic_p2g_brick(ctx, order, entity, dtype, *mesh, np, *x, *m_p, *c_p,
+                                *m_i, *mu_i, *bins);
+        if (rc != PIC_ERR_UNSUPPORTED) return rc;
+    }
+    TimedLaunch tl(ctx, "apic_p2g");
+    const uint32_t* perm = bins ? bins->perm : nullptr;
+    if (dtype == PIC_F64) {
+        P2GScatterLaunch<double> L{make_args<double>(ctx, entity, *mesh, np, x, m_p, c_p, m_i,
+                                                     *mu_i, perm),
+                                   ctx->stream};
+        dispatch_order_stag<double>(order, stag, L);
+    } else {
+        P2GScatterLaunch<float> L{make_args<float>(ctx, entity, *mesh, np, x, m_p, c_p, m_i,
+                                                   *mu_i, perm),
+                                  ctx->stream};
+        dispatch_order_stag<float>(order, stag, L);
+    }
+    return launch_status();
+}
+
+extern "C" int pic_apic_g2p(pic_context* ctx, int order, int entity, int dtype,
+                            const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                            const pic_grid* u_i, const pic_pfield* c_p,
+                            const pic_bins* bins)
+{
+    if (!ctx || !mesh || !(mesh->h > 0.0) || order < 1 || order > 3 ||
+        !valid_entity(entity) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
+        np > int64_t(UINT32_MAX) || !valid_grid(u_i))
+        return PIC_ERR_ARG;
+    if (np > 0 && (!valid_pfield(x) || !valid_pfield(c_p))) return PIC_ERR_ARG;
+    if (bins && !bins->perm) return PIC_ERR_ARG;
+    if (np == 0) return PIC_OK;
+    DeviceGuard g(ctx->device);
+    if (!g.ok) return PIC_ERR_HIP;
+    const bool stag = stag_dim(entity) >= 0;
+    if (bins) {
+        int rc = apic_g2p_brick(ctx, order, entity, dtype, *mesh, np, *x, *u_i, *c_p, *bins);
+        if (rc != PIC_ERR_UNSUPPORTED) return rc;
+    }
+    TimedLaunch tl(ctx, "apic_g2p");
+    const uint32_t* perm = bins ? bins->perm : nullptr;
+    if (dtype == PIC_F64) {
+        G2PGatherLaunch<double> L{
+            make_args<double>(ctx, entity, *mesh, np, x, nullptr, c_p, nullptr, *u_i, perm),
+            ctx->stream};
+        dispatch_order_stag<double>(order, stag, L);
+    } else {
+        G2PGatherLaunch<float> L{
+            make_args<float>(ctx, entity, *mesh, np, x, nullptr, c_p, nullptr, *u_i, perm),
+            ctx->stream};
+        dispatch_order_stag<float>(order, stag, L);
+    }
+    return launch_status();
+}

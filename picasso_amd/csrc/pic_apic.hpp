@@ -1,0 +1,198 @@
+// Per-particle APIC arithmetic shared by the scatter and brick kernels.
+// Restates Picasso_APIC.hpp: p2g orders 2/3 (:55-140 collocated, :148-232
+// staggered), p2g order 1 (:240-319, :327-402), g2p (:407-472, :477-544).
+#pragma once
+
+#include "pic_common.hpp"
+#include "pic_internal.hpp"
+
+namespace pic {
+
+template <typename T>
+struct ApicArgs {
+    Lattice lat;
+    int64_t np;
+    const uint32_t* perm;  // sorted slot -> particle, or null (identity)
+    PField<T> x, m, c;     // positions (3), mass (1), c_p (12)
+    GView<T> gm, gmu;      // grid mass, grid momentum (or velocity for g2p)
+    int sdim;              // staggered component (entity dim)
+    int* err;
+};
+
+// Particle state for p2g.  Staggered transfers keep only row `sdim` of
+// u_p and B_p, stored at index 0.
+template <typename T>
+struct ApicParticle {
+    T x[3];
+    T m;
+    T c0[3];
+    T B[3][3];
+};
+
+template <bool STAG, typename T>
+__device__ __forceinline__ void load_apic_particle(const ApicArgs<T>& a, int64_t p,
+                                                   ApicParticle<T>& pt)
+{
+    const int64_t xb = a.x.base(p);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pt.x[d] = a.x.at(xb, d);
+    pt.m = a.m.at(a.m.base(p), 0);
+    const int64_t cb = a.c.base(p);
+    // B_p is sliced from c_p and transposed: B(d, a) = c_p(1 + a, d)
+    // (Picasso_APIC.hpp:94-101).
+    if constexpr (STAG) {
+        const int d = a.sdim;
+        pt.c0[0] = a.c.at(cb, d);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) pt.B[0][q] = a.c.at(cb, (1 + q) * 3 + d);
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            pt.c0[d] = a.c.at(cb, d);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) pt.B[d][q] = a.c.at(cb, (1 + q) * 3 + d);
+        }
+    }
+}
+
+// Contributions of one particle to its stencil nodes, handed to
+// emit(i, j, k, mu[ncomp], m) in the reference's i-outer, k-inner order.
+template <int ORDER, bool STAG, typename T, typename Emit>
+__device__ __forceinline__ void apic_p2g_particle(const ApicArgs<T>& a,
+                                                  const ApicParticle<T>& pt,
+                                                  const Spline<ORDER, T>& sd, Emit&& emit)
+{
+    constexpr int NK = Knots<ORDER>::NK;
+    constexpr int NC = STAG ? 1 : 3;
+    if constexpr (ORDER >= 2) {
+        // APIC::inertialScaling (Picasso_APIC.hpp:33-47).
+        const T hh = (T)a.lat.h;
+        const T Dinv = T(ORDER == 2 ? 4.0 : 3.0) / (hh * hh);
+        T DB[NC][3];
+#pragma unroll
+        for (int d = 0; d < NC; ++d)
+#pragma unroll
+            for (int q = 0; q < 3; ++q) DB[d][q] = Dinv * pt.B[d][q];
+#pragma unroll
+        for (int i = 0; i < NK; ++i)
+#pragma unroll
+            for (int j = 0; j < NK; ++j) {
+                const T wij = sd.w[0][i] * sd.w[1][j];
+#pragma unroll
+                for (int k = 0; k < NK; ++k) {
+                    const T wm = wij * sd.w[2][k] * pt.m;
+                    T mu[NC];
+#pragma unroll
+                    for (int d = 0; d < NC; ++d)
+                        mu[d] = wm * (pt.c0[d] + (DB[d][0] * sd.d[0][i] + DB[d][1] * sd.d[1][j] +
+                                                  DB[d][2] * sd.d[2][k]));
+                    emit(sd.s0[0] + i, sd.s0[1] + j, sd.s0[2] + k, mu, wm);
+                }
+            }
+    } else {
+        // Gradient form, no inertial scaling (Picasso_APIC.hpp:287-318).
+#pragma unroll
+        for (int i = 0; i < NK; ++i)
+#pragma unroll
+            for (int j = 0; j < NK; ++j)
+#pragma unroll
+                for (int k = 0; k < NK; ++k) {
+                    const T wm = sd.w[0][i] * sd.w[1][j] * sd.w[2][k] * pt.m;
+                    const T gm[3] = {sd.g[0][i] * sd.w[1][j] * sd.w[2][k] * pt.m,
+                                     sd.w[0][i] * sd.g[1][j] * sd.w[2][k] * pt.m,
+                                     sd.w[0][i] * sd.w[1][j] * sd.g[2][k] * pt.m};
+                    T mu[NC];
+#pragma unroll
+                    for (int d = 0; d < NC; ++d)
+                        mu[d] = wm * pt.c0[d] +
+                                (pt.B[d][0] * gm[0] + pt.B[d][1] * gm[1] + pt.B[d][2] * gm[2]);
+                    emit(sd.s0[0] + i, sd.s0[1] + j, sd.s0[2] + k, mu, wm);
+                }
+    }
+}
+
+// g2p of one particle: fetch(i, j, k, ue[ncomp]) reads the grid; the 12
+// components of c_p are returned in cp (row-major 4x3).  A staggered g2p
+// leaves zeros in the other components (Picasso_APIC.hpp:505-543).
+template <int ORDER, bool STAG, typename T, typename Fetch>
+__device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
+                                                  const Spline<ORDER, T>& sd, Fetch&& fetch,
+                                                  T cp[12])
+{
+    constexpr int NK = Knots<ORDER>::NK;
+    constexpr int NC = STAG ? 1 : 3;
+    T up[NC];
+    T B[NC][3];
+#pragma unroll
+    for (int d = 0; d < NC; ++d) {
+        up[d] = T(0);
+        B[d][0] = B[d][1] = B[d][2] = T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < NK; ++i)
+#pragma unroll
+        for (int j = 0; j < NK; ++j) {
+            const T wij = sd.w[0][i] * sd.w[1][j];
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                const T wip = wij * sd.w[2][k];
+                T ue[NC];
+                fetch(sd.s0[0] + i, sd.s0[1] + j, sd.s0[2] + k, ue);
+#pragma unroll
+                for (int d = 0; d < NC; ++d) {
+                    const T wu = wip * ue[d];
+                    up[d] += wu;
+                    B[d][0] += wu * sd.d[0][i];
+                    B[d][1] += wu * sd.d[1][j];
+                    B[d][2] += wu * sd.d[2][k];
+                }
+            }
+        }
+    if constexpr (STAG) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const bool on = d == a.sdim;
+            cp[d] = on ? up[0] : T(0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) cp[(1 + q) * 3 + d] = on ? B[0][q] : T(0);
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            cp[d] = up[d];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) cp[(1 + q) * 3 + d] = B[d][q];
+        }
+    }
+}
+
+// Order/staggering dispatch to a functor with a templated call operator.
+template <typename T, typename Launch>
+inline void dispatch_order_stag(int order, bool stag, Launch& L)
+{
+    switch (order) {
+    case 1:
+        if (stag) L.template operator()<1, true, T>();
+        else L.template operator()<1, false, T>();
+        break;
+    case 2:
+        if (stag) L.template operator()<2, true, T>();
+        else L.template operator()<2, false, T>();
+        break;
+    default:
+        if (stag) L.template operator()<3, true, T>();
+        else L.template operator()<3, false, T>();
+        break;
+    }
+}
+
+// Brick scheme (pic_apic_brick.hip).  Return PIC_ERR_UNSUPPORTED to fall
+// back to the scatter/gather kernels.
+int apic_p2g_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
+                   int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
+                   const pic_grid& gm, const pic_grid& gmu, const pic_bins& bins);
+int apic_g2p_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
+                   int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_pfield& c,
+                   const pic_bins& bins);
+
+}  // namespace pic

@@ -1,0 +1,93 @@
+// Host-side context, argument validation and launch bookkeeping shared by the
+// translation units of the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "picasso_amd.h"
+
+struct pic_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int* d_err = nullptr;  // device-side error word (ERRBIT_*)
+
+    // Growable device scratch, reused across calls on this context.
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+
+    // Optional per-launch event timing.
+    bool timing = false;
+    struct Rec {
+        std::string name;
+        hipEvent_t start, stop;
+    };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> event_pool;
+};
+
+namespace pic {
+
+#define PIC_HIP_CHECK(expr)                                \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return PIC_ERR_HIP;          \
+    } while (0)
+
+// Make the context's device current for the duration of a call.
+struct DeviceGuard {
+    int prev = -1;
+    bool ok = false;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) return;
+        ok = hipSetDevice(dev) == hipSuccess;
+    }
+    ~DeviceGuard()
+    {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+// Ensure at least `bytes` of scratch; returns nullptr on failure.
+void* scratch(pic_context* ctx, size_t bytes);
+
+// Launch bracketing for optional timing.
+struct TimedLaunch {
+    pic_context* ctx;
+    size_t idx;
+    TimedLaunch(pic_context* c, const char* name);
+    ~TimedLaunch();
+};
+
+// Launch error check after a kernel launch.
+inline int launch_status()
+{
+    return hipGetLastError() == hipSuccess ? PIC_OK : PIC_ERR_HIP;
+}
+
+inline bool valid_pfield(const pic_pfield* f)
+{
+    return f && f->data && f->vlen_log2 >= 0 && f->vlen_log2 < 31;
+}
+
+inline bool valid_grid(const pic_grid* g)
+{
+    return g && g->data && g->n[0] > 0 && g->n[1] > 0 && g->n[2] > 0;
+}
+
+inline bool valid_entity(int e) { return e >= PIC_NODE && e <= PIC_EDGE_K; }
+
+inline unsigned blocks_for(int64_t n, int threads)
+{
+    return (unsigned)((n + threads - 1) / threads);
+}
+
+// Zero a grid window on the stream (used by PIC_P2G_ZERO_FIRST).
+int zero_grid(pic_context* ctx, int dtype, const pic_grid& g, int ncomp);
+
+}  // namespace pic

@@ -1,0 +1,290 @@
+"""HIP kernels against the CPU oracle on identical seeded inputs.
+
+Bars: FP64 within 1e-12 relative (||.||_inf / ||ref||_inf; atomic summation
+order differs), cell keys / permutations / bin offsets bit-exact, mass and
+momentum conserved to round-off, FP32 paths within 1e-5 of the FP64 oracle
+evaluated on the same (FP32-rounded) inputs.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from oracle import orpic  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+F64_TOL = 1e-12
+F32_TOL = 1e-5
+
+
+@pytest.fixture(scope="module")
+def pa():
+    import picasso_amd
+    return picasso_amd
+
+
+@pytest.fixture(scope="module")
+def wl0():
+    from picasso_amd import synth
+    return synth.apic_uniform(32, 8)
+
+
+def _node_grids(pa, lo, n, dtype, ncomp=3):
+    import gpu_util as gu
+    shp = tuple(int(v) for v in n)
+    gm = pa.GridField(torch.zeros(shp, dtype=dtype, device=gu.DEV), tuple(int(v) for v in lo))
+    gmu = pa.GridField(torch.zeros(shp + (ncomp,), dtype=dtype, device=gu.DEV),
+                       tuple(int(v) for v in lo))
+    return gm, gmu
+
+
+def _oracle_step(wl, x, m, c, order=2):
+    lo, n = wl.node_window
+    gm = np.zeros(tuple(n))
+    gmu = np.zeros(tuple(n) + (3,))
+    orpic.apic_p2g(order, orpic.NODE, wl.low, wl.h, lo, n, x, m, c, gm, gmu)
+    gu = orpic.grid_normalize(gm, gmu)
+    cout = orpic.apic_g2p(order, orpic.NODE, wl.low, wl.h, lo, n, x, gu)
+    return gm, gmu, gu, cout
+
+
+@pytest.mark.parametrize("layout", ["soa", "aos"])
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_apic_step_f64_uniform(pa, wl0, layout, order):
+    """p2g -> normalise -> g2p on the 32^3 x 8 ppc workload, FP64."""
+    import gpu_util as gu
+    lo, n = wl0.node_window
+    put = gu.soa if layout == "soa" else gu.dev
+    x = put(wl0.x)
+    m = put(wl0.m[:, None])
+    c = put(wl0.c.reshape(-1, 12))
+    mesh = pa.Mesh(wl0.low, wl0.h)
+    sd = pa.create_spline(pa.FieldLocation.Node, order, mesh, x)
+    gm, gmu = _node_grids(pa, lo, n, torch.float64)
+    gu_ = pa.GridField(torch.empty_like(gmu.tensor), gmu.lo)
+    pa.APIC.p2g(gu.ctx(), m, c, gm, gmu, sd, zero_first=True)
+    pa.grid_normalize(gu.ctx(), gm, gmu, gu_)
+    pa.APIC.g2p(gu.ctx(), gu_, c, sd)
+    gu.ctx().synchronize()
+
+    rm, rmu, ru, rc = _oracle_step(wl0, wl0.x, wl0.m, wl0.c, order)
+    assert gu.rel_inf(gu.host(gm.tensor), rm) < F64_TOL
+    assert gu.rel_inf(gu.host(gmu.tensor), rmu) < F64_TOL
+    assert gu.rel_inf(gu.host(gu_.tensor), ru) < F64_TOL
+    cc = gu.host(c).reshape(-1, 4, 3)
+    assert gu.rel_inf(cc[:, 0], rc[:, 0]) < F64_TOL
+    assert gu.rel_inf(cc[:, 1:], rc[:, 1:]) < F64_TOL
+    # conservation
+    assert abs(gu.host(gm.tensor).sum() - wl0.m.sum()) <= 1e-13 * wl0.m.sum()
+    mom = (wl0.m[:, None] * wl0.c[:, 0, :]).sum(0)
+    scale = (wl0.m[:, None] * np.abs(wl0.c[:, 0, :])).sum(0)
+    np.testing.assert_array_less(np.abs(gu.host(gmu.tensor).reshape(-1, 3).sum(0) - mom),
+                                 1e-12 * scale)
+
+
+@pytest.mark.parametrize("order", [2])
+def test_apic_step_f32_uniform(pa, wl0, order):
+    """FP32 path against the FP64 oracle on the FP32-rounded inputs."""
+    import gpu_util as gu
+    lo, n = wl0.node_window
+    x32 = wl0.x.astype(np.float32)
+    m32 = wl0.m.astype(np.float32)
+    c32 = wl0.c.astype(np.float32)
+    x = gu.soa(x32, torch.float32)
+    m = gu.soa(m32[:, None], torch.float32)
+    c = gu.soa(c32.reshape(-1, 12), torch.float32)
+    mesh = pa.Mesh(wl0.low, wl0.h)
+    sd = pa.create_spline(pa.FieldLocation.Node, order, mesh, x)
+    gm, gmu = _node_grids(pa, lo, n, torch.float32)
+    gu_ = pa.GridField(torch.empty_like(gmu.tensor), gmu.lo)
+    pa.APIC.p2g(gu.ctx(), m, c, gm, gmu, sd, zero_first=True)
+    pa.grid_normalize(gu.ctx(), gm, gmu, gu_, rel_mass_eps=1e-6)
+    pa.APIC.g2p(gu.ctx(), gu_, c, sd)
+    gu.ctx().synchronize()
+    rm, rmu, ru, rc = _oracle_step(wl0, x32.astype(np.float64), m32.astype(np.float64),
+                                   c32.astype(np.float64), order)
+    assert gu.rel_inf(gu.host(gm.tensor), rm) < F32_TOL
+    assert gu.rel_inf(gu.host(gmu.tensor), rmu) < F32_TOL
+    cc = gu.host(c).reshape(-1, 4, 3)
+    assert gu.rel_inf(cc[:, 0], rc[:, 0]) < F32_TOL
+    assert gu.rel_inf(cc[:, 1:], rc[:, 1:]) < 1e-4
+
+
+@pytest.mark.parametrize("entity", range(8))
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_apic_entities_random(pa, entity, order):
+    """All entity lattices, random particles, FP64, vs the oracle."""
+    import gpu_util as gu
+    rng = np.random.default_rng(100 * order + entity)
+    h = 0.25
+    low = np.array([-1.0, -0.5, -0.75])
+    x = low + h * rng.uniform(4, 12, size=(500, 3))
+    lo = np.array([1, 2, 0], np.int32)
+    n = np.array([16, 15, 17], np.int32)
+    m = rng.uniform(0.5, 1.5, 500)
+    c = rng.normal(size=(500, 4, 3))
+    stag = entity >= 2
+    rm = np.zeros(tuple(n))
+    rmu = np.zeros(tuple(n) + (() if stag else (3,)))
+    orpic.apic_p2g(order, entity, low, h, lo, n, x, m, c, rm, rmu)
+    gfield = rng.normal(size=tuple(n) + (() if stag else (3,)))
+    rc = orpic.apic_g2p(order, entity, low, h, lo, n, x, gfield)
+
+    mesh = pa.Mesh(low, h)
+    sd = pa.create_spline(entity, order, mesh, gu.dev(x))
+    gm = pa.GridField(gu.dev(np.zeros(tuple(n))), tuple(lo))
+    gmu = pa.GridField(gu.dev(np.zeros(rmu.shape)), tuple(lo))
+    pa.APIC.p2g(gu.ctx(), gu.dev(m[:, None]), gu.dev(c.reshape(-1, 12)), gm, gmu, sd)
+    cg = gu.dev(np.zeros((500, 12)))
+    pa.APIC.g2p(gu.ctx(), pa.GridField(gu.dev(gfield), tuple(lo)), cg, sd)
+    gu.ctx().synchronize()
+    assert gu.rel_inf(gu.host(gm.tensor), rm) < F64_TOL
+    assert gu.rel_inf(gu.host(gmu.tensor), rmu) < F64_TOL
+    assert gu.rel_inf(gu.host(cg).reshape(-1, 4, 3), rc) < F64_TOL
+
+
+@pytest.mark.parametrize("brick", [(1, 1, 1), (4, 4, 4), (8, 8, 8), (2, 4, 8)])
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_bin_sort_bit_exact(pa, wl0, brick, dt):
+    import gpu_util as gu
+    tdt = torch.float64 if dt == "f64" else torch.float32
+    xn = wl0.x if dt == "f64" else wl0.x.astype(np.float32)
+    x = gu.soa(xn, tdt)
+    cell_lo, ncell = wl0.cell_region
+    res = pa.bin_sort(gu.ctx(), pa.Mesh(wl0.low, wl0.h), x, tuple(cell_lo), tuple(ncell), brick)
+    gu.ctx().synchronize()
+    keys, _ = orpic.cell_keys(xn.astype(np.float64), wl0.low, wl0.h, cell_lo, ncell, brick)
+    nb = pa.bin_count(tuple(ncell), brick)
+    bvol = int(np.prod(brick))
+    perm, kstart = orpic.bin_sort(keys, nb * bvol)
+    np.testing.assert_array_equal(gu.host(res.keys).view(np.uint32), keys)
+    np.testing.assert_array_equal(gu.host(res.perm).view(np.uint32), perm)
+    np.testing.assert_array_equal(gu.host(res.bin_start).view(np.uint32), kstart[::bvol])
+
+
+def test_binned_p2g_matches_unbinned(pa, wl0):
+    import gpu_util as gu
+    lo, n = wl0.node_window
+    x = gu.soa(wl0.x)
+    m = gu.soa(wl0.m[:, None])
+    c = gu.soa(wl0.c.reshape(-1, 12))
+    mesh = pa.Mesh(wl0.low, wl0.h)
+    cell_lo, ncell = wl0.cell_region
+    bins = pa.bin_sort(gu.ctx(), mesh, x, tuple(cell_lo), tuple(ncell), (8, 8, 8))
+    gm, gmu = _node_grids(pa, lo, n, torch.float64)
+    pa.APIC.p2g(gu.ctx(), m, c, gm, gmu, pa.create_spline(0, 2, mesh, x, bins))
+    gm2, gmu2 = _node_grids(pa, lo, n, torch.float64)
+    pa.APIC.p2g(gu.ctx(), m, c, gm2, gmu2, pa.create_spline(0, 2, mesh, x))
+    c_b = torch.zeros_like(c)
+    c_u = torch.zeros_like(c)
+    u = pa.GridField(torch.randn_like(gmu.tensor), gmu.lo)
+    pa.APIC.g2p(gu.ctx(), u, c_b, pa.create_spline(0, 2, mesh, x, bins))
+    pa.APIC.g2p(gu.ctx(), u, c_u, pa.create_spline(0, 2, mesh, x))
+    gu.ctx().synchronize()
+    assert gu.rel_inf(gu.host(gm.tensor), gu.host(gm2.tensor)) < F64_TOL
+    assert gu.rel_inf(gu.host(gmu.tensor), gu.host(gmu2.tensor)) < F64_TOL
+    assert gu.rel_inf(gu.host(c_b), gu.host(c_u)) < F64_TOL
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("ncomp", [1, 3])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_polypic_random(pa, order, ncomp, dtype):
+    import gpu_util as gu
+    rng = np.random.default_rng(7 * order + ncomp)
+    h = 0.125
+    low = np.zeros(3)
+    P = 300
+    x = low + h * rng.uniform(4, 12, size=(P, 3))
+    lo = np.array([0, 0, 0], np.int32)
+    n = np.array([18, 18, 18], np.int32)
+    nm = orpic.polypic_nmode(order)
+    m = rng.uniform(0.5, 1.5, P)
+    u = rng.normal(size=(P, nm, 3))
+    cm = rng.normal(size=(P, nm, ncomp))
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    if dtype == "f32":
+        x, m, u, cm = (a.astype(np.float32).astype(np.float64) for a in (x, m, u, cm))
+    dt = 1e-3
+    rmu = np.zeros(tuple(n) + (ncomp,))
+    rm = np.zeros(tuple(n))
+    orpic.polypic_p2g(order, orpic.NODE, low, h, lo, n, x, m, u, cm, dt, rmu, rm)
+    gfield = rng.normal(size=tuple(n) + (ncomp,))
+    if dtype == "f32":
+        gfield = gfield.astype(np.float32).astype(np.float64)
+    rc = orpic.polypic_g2p(order, orpic.NODE, low, h, lo, n, x, gfield, ncomp)
+
+    mesh = pa.Mesh(low, h)
+    sd = pa.create_spline(0, order, mesh, gu.dev(x, tdt))
+    gmu = pa.GridField(gu.dev(np.zeros(rmu.shape), tdt))
+    gm = pa.GridField(gu.dev(np.zeros(rm.shape), tdt))
+    pa.PolyPIC.p2g(gu.ctx(), gu.dev(m[:, None], tdt), gu.dev(u.reshape(P, -1), tdt),
+                   gu.dev(cm.reshape(P, -1), tdt), gmu, gm, dt, sd, ncomp=ncomp)
+    cg = gu.dev(np.zeros((P, nm * ncomp)), tdt)
+    pa.PolyPIC.g2p(gu.ctx(), pa.GridField(gu.dev(gfield, tdt)), cg, sd, ncomp=ncomp)
+    gu.ctx().synchronize()
+    tol = F64_TOL if dtype == "f64" else F32_TOL
+    assert gu.rel_inf(gu.host(gm.tensor), rm) < tol
+    assert gu.rel_inf(gu.host(gmu.tensor), rmu) < tol
+    assert gu.rel_inf(gu.host(cg).reshape(P, nm, ncomp), rc) < tol * 10
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_gradu_and_update_F(pa, order):
+    import gpu_util as gu
+    rng = np.random.default_rng(order)
+    h = 0.125
+    low = np.zeros(3)
+    P = 400
+    x = low + h * rng.uniform(4, 12, size=(P, 3))
+    lo = np.array([0, 0, 0], np.int32)
+    n = np.array([18, 18, 18], np.int32)
+    gfield = rng.normal(size=tuple(n) + (3,))
+    rL = orpic.g2p_gradu(order, orpic.NODE, low, h, lo, n, x, gfield)
+    F0 = np.eye(3)[None] + 0.01 * rng.normal(size=(P, 3, 3))
+    rF, rdet, rinv = orpic.update_F(rL, 1e-3, F0)
+
+    sd = pa.create_spline(0, order, pa.Mesh(low, h), gu.dev(x))
+    L = gu.dev(np.zeros((P, 9)))
+    pa.g2p_gradu(gu.ctx(), pa.GridField(gu.dev(gfield)), L, sd)
+    F = gu.dev(F0.reshape(P, 9))
+    det = gu.dev(np.zeros((P, 1)))
+    inv = gu.dev(np.zeros((P, 9)))
+    pa.update_F(gu.ctx(), L, 1e-3, F, det, inv)
+    gu.ctx().synchronize()
+    assert gu.rel_inf(gu.host(L).reshape(P, 3, 3), rL) < F64_TOL
+    assert gu.rel_inf(gu.host(F).reshape(P, 3, 3), rF) < F64_TOL
+    assert gu.rel_inf(gu.host(det)[:, 0], rdet) < F64_TOL
+    assert gu.rel_inf(gu.host(inv).reshape(P, 3, 3), rinv) < F64_TOL
+
+
+def test_out_of_window_is_reported(pa):
+    import gpu_util as gu
+    x = gu.dev([[0.01, 0.5, 0.5]])
+    mesh = pa.Mesh([0.0, 0.0, 0.0], 0.1)
+    gm = pa.GridField(gu.dev(np.zeros((10, 10, 10))))
+    gmu = pa.GridField(gu.dev(np.zeros((10, 10, 10, 3))))
+    pa.APIC.p2g(gu.ctx(), gu.dev([[1.0]]), gu.dev(np.ones((1, 12))), gm, gmu,
+                pa.create_spline(0, 2, mesh, x))
+    with pytest.raises(pa.PicError) as ei:
+        gu.ctx().synchronize()
+    assert ei.value.code == -2
+    # nothing was written and the error is cleared
+    assert gu.host(gm.tensor).sum() == 0.0
+    gu.ctx().synchronize()
+
+
+def test_empty_inputs(pa):
+    import gpu_util as gu
+    x = gu.dev(np.zeros((0, 3)))
+    mesh = pa.Mesh([0.0, 0.0, 0.0], 0.1)
+    gm = pa.GridField(gu.dev(np.ones((4, 4, 4))))
+    gmu = pa.GridField(gu.dev(np.ones((4, 4, 4, 3))))
+    pa.APIC.p2g(gu.ctx(), gu.dev(np.zeros((0, 1))), gu.dev(np.zeros((0, 12))), gm, gmu,
+                pa.create_spline(0, 2, mesh, x), zero_first=True)
+    gu.ctx().synchronize()
+    assert gu.host(gm.tensor).sum() == 0.0 and gu.host(gmu.tensor).sum() == 0.0
+    res = pa.bin_sort(gu.ctx(), mesh, x, (0, 0, 0), (4, 4, 4), (2, 2, 2))
+    gu.ctx().synchronize()
+    assert np.all(gu.host(res.bin_start) == 0)
