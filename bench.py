@@ -1,0 +1,242 @@
+"""Throughput of the APIC particle<->grid transfer step on one MI355X.
+
+Workload: 256^3 cells on the unit cube, quadratic B-splines on nodes,
+particles at 8 per cell in the cells whose centre lies within 0.3 of the
+cube centre (~15.2M particles), v = rigid rotation about z plus N(0, 0.05),
+B ~ N(0, 0.1), shuffled.  One step = cell-bin sort -> p2g (grid zeroed
+first) -> grid normalisation -> g2p, with inputs resident in HBM.
+
+The headline value is the FP32 (compensated p2g) step in particles/s; the
+plain FP64 step is timed beside it, and the FP32 grid momentum is compared
+with the FP64 step on the same FP32-rounded inputs.  The roofline object
+covers the kernel family with the largest share of the step, from HIP
+events recorded around every launch on the stream the kernels run on.  The
+CPU baseline is the oracle's OpenMP FP64 step on the box's host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256] [--ppc 8]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("APIC p2g+g2p particles/s @256³×8ppc, FP32-comp vs FP64 at stated err; "
+          "HBM GB/s")
+HBM_PEAK_GBS = 8000.0
+BRICK = (8, 8, 8)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+class Step:
+    """Device buffers and one APIC step for one precision."""
+
+    def __init__(self, pa, ctx, wl, dtype, x, m, c):
+        import torch
+        self.pa, self.ctx, self.torch = pa, ctx, torch
+        self.dtype = dtype
+        dev = "cuda:0"
+        P = wl.np
+        # SoA particle storage: [q, P] tensors viewed as [P, q].
+        self.x = torch.as_tensor(np.ascontiguousarray(x.T), dtype=dtype, device=dev).t()
+        self.m = torch.as_tensor(m[None, :], dtype=dtype, device=dev).t()
+        self.c = torch.as_tensor(np.ascontiguousarray(c.reshape(P, 12).T), dtype=dtype,
+                                 device=dev).t()
+        self.c0 = self.c.clone()
+        lo, n = wl.node_window
+        nn = tuple(int(v) for v in n)
+        self.gm = pa.GridField(torch.zeros(nn, dtype=dtype, device=dev))
+        self.gmu = pa.GridField(torch.zeros((3,) + nn, dtype=dtype, device=dev).permute(1, 2, 3, 0))
+        self.gu = pa.GridField(torch.zeros((3,) + nn, dtype=dtype, device=dev).permute(1, 2, 3, 0))
+        self.mesh = pa.Mesh(wl.low, wl.h)
+        cell_lo, ncell = wl.cell_region
+        self.cell_lo, self.ncell = tuple(int(v) for v in cell_lo), tuple(int(v) for v in ncell)
+        nb = pa.bin_count(self.ncell, BRICK)
+        self.bins = pa.BinSort(torch.empty(P, dtype=torch.int32, device=dev),
+                               torch.empty(P, dtype=torch.int32, device=dev),
+                               torch.empty(nb + 1, dtype=torch.int32, device=dev),
+                               self.cell_lo, self.ncell, BRICK)
+
+    def restore(self):
+        self.c.copy_(self.c0)
+
+    def run(self):
+        pa, ctx = self.pa, self.ctx
+        pa.bin_sort(ctx, self.mesh, self.x, self.cell_lo, self.ncell, BRICK, out=self.bins)
+        sd = pa.create_spline(pa.FieldLocation.Node, 2, self.mesh, self.x, self.bins)
+        pa.APIC.p2g(ctx, self.m, self.c, self.gm, self.gmu, sd, zero_first=True)
+        pa.grid_normalize(ctx, self.gm, self.gmu, self.gu,
+                          rel_mass_eps=1e-6 if self.dtype is self.torch.float32 else 0.0)
+        pa.APIC.g2p(ctx, self.gu, self.c, sd)
+
+
+def time_steps(step, steps, warmup, ctx, families):
+    import torch
+    for _ in range(warmup):
+        step.run()
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step.run()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ctx.set_timing(False)
+    ctx.synchronize()
+    kt = {}
+    for f in families:
+        ms, n = ctx.kernel_time(f)
+        if n:
+            kt[f] = {"ms_total": ms, "launches": n, "ms_avg": ms / n}
+    ctx.reset_timing()
+    return (t1 - t0) * 1e3 / steps, kt
+
+
+def cpu_baseline(wl, reps=3):
+    """Oracle OpenMP FP64 step (atomic p2g) on the host cores."""
+    from oracle import orpic
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    lo, n = wl.node_window
+    x = np.ascontiguousarray(wl.x)
+    c = np.ascontiguousarray(wl.c.copy())
+    orpic.apic_step_omp(2, wl.low, wl.h, lo, n, x, wl.m, c, mode=1, nthreads=threads)  # warm-up
+    ts = []
+    for _ in range(reps):
+        c[:] = wl.c
+        t0 = time.perf_counter()
+        orpic.apic_step_omp(2, wl.low, wl.h, lo, n, x, wl.m, c, mode=1, nthreads=threads)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    return {"value": wl.np / t, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": f"full workload ({wl.np} particles), FP64 p2g(atomic)->normalise->g2p, "
+                      f"median of {reps} after 1 warm-up, {t * 1e3:.1f} ms/step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--ppc", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the transfer path is built and timed on a single "
+              "MI355X; multi-GPU runs are not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import picasso_amd as pa
+    from picasso_amd import synth
+
+    t0 = time.perf_counter()
+    wl = synth.apic_sphere(args.n, args.ppc)
+    P = wl.np
+    log(f"workload {wl.name}: {P} particles ({time.perf_counter() - t0:.1f} s)")
+    n_touched = synth.touched_nodes(wl.x.astype(np.float32).astype(np.float64), wl.low, wl.h, 2)
+    log(f"touched nodes {n_touched}")
+
+    ctx = pa.Context(0)
+    x32, m32, c32 = wl.x.astype(np.float32), wl.m.astype(np.float32), wl.c.astype(np.float32)
+    families = ["bin_sort", "zero_grid", "apic_p2g", "max_mass", "normalize", "apic_g2p"]
+
+    s32 = Step(pa, ctx, wl, torch.float32, x32, m32, c32)
+    ms32, kt32 = time_steps(s32, args.steps, args.warmup, ctx, families)
+    log(f"fp32: {ms32:.3f} ms/step, kernels {json.dumps(kt32)}")
+
+    s64 = Step(pa, ctx, wl, torch.float64, wl.x, wl.m, wl.c)
+    ms64, kt64 = time_steps(s64, args.steps, args.warmup, ctx, families)
+    log(f"fp64: {ms64:.3f} ms/step, kernels {json.dumps(kt64)}")
+    del s64
+
+    # Error of the FP32 step against the FP64 step on the same rounded inputs.
+    ref = Step(pa, ctx, wl, torch.float64, x32.astype(np.float64), m32.astype(np.float64),
+               c32.astype(np.float64))
+    ref.run()
+    s32.restore()
+    s32.run()
+    ctx.synchronize()
+
+    def rel(a, b):
+        a = a.double()
+        b = b.double()
+        return float((a - b).abs().max() / b.abs().max())
+
+    err = {"grid_mass": rel(s32.gm.tensor, ref.gm.tensor),
+           "grid_momentum": rel(s32.gmu.tensor, ref.gmu.tensor),
+           "particle_velocity": rel(s32.c[:, :3], ref.c[:, :3]),
+           "particle_B": rel(s32.c[:, 3:], ref.c[:, 3:])}
+    log(f"fp32 error vs fp64: {err}")
+    del ref
+
+    s_alg32 = 4
+    B_alg32 = P * 31 * s_alg32 + n_touched * 14 * s_alg32
+    per_kernel_bytes = {
+        "apic_p2g": P * 16 * s_alg32 + n_touched * 4 * s_alg32,
+        "normalize": n_touched * 7 * s_alg32,
+        "apic_g2p": P * 15 * s_alg32 + n_touched * 3 * s_alg32,
+    }
+    dom = max(kt32, key=lambda f: kt32[f]["ms_total"])
+    roof = None
+    if dom in per_kernel_bytes:
+        gbs = per_kernel_bytes[dom] / (kt32[dom]["ms_avg"] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": dom, "achieved": gbs, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                "alg_bytes_per_launch": per_kernel_bytes[dom]}
+    else:
+        roof = {"bound": "hbm", "kernel": dom, "achieved": None, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": None, "traffic": None,
+                "note": "dominant kernel has no algorithmic-byte definition (sort overhead)"}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(wl)
+        log(f"cpu baseline {cpu}")
+
+    step_gbs = B_alg32 / (ms32 * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": P / (ms32 * 1e-3),
+        "unit": "particles/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms32,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded SplitMix64), inputs resident in HBM",
+        "config": {"workload": f"APIC sphere r=0.3, {args.n}^3 cells x {args.ppc} ppc, "
+                               "quadratic node spline, cell-bin sort every step",
+                   "particles": P, "cells": args.n, "ppc": args.ppc,
+                   "touched_nodes": n_touched, "brick": list(BRICK)},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "fp64": {"particles_per_s": P / (ms64 * 1e-3), "ms_per_step": ms64,
+                 "kernels": kt64},
+        "fp32_over_fp64": ms64 / ms32,
+        "fp32_error_vs_fp64": err,
+        "step_alg_bytes": B_alg32,
+        "step_alg_GBs": step_gbs,
+        "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
+        "kernels": kt32,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
