@@ -57,20 +57,31 @@ class Step:
         self.gmu = pa.GridField(torch.zeros((3,) + nn, dtype=dtype, device=dev).permute(1, 2, 3, 0))
         self.gu = pa.GridField(torch.zeros((3,) + nn, dtype=dtype, device=dev).permute(1, 2, 3, 0))
         self.mesh = pa.Mesh(wl.low, wl.h)
+        # Bins are the node-centred dual cells: the knot base of quadratic
+        # splines on nodes, so each bin's particles share one stencil.
         cell_lo, ncell = wl.cell_region
-        self.cell_lo, self.ncell = tuple(int(v) for v in cell_lo), tuple(int(v) for v in ncell)
-        nb = pa.bin_count(self.ncell, BRICK)
+        self.geom = pa.Binning(tuple(int(v) for v in cell_lo), tuple(int(v) + 1 for v in ncell),
+                               BRICK, 1)
+        nb = pa.bin_count(self.geom)
         self.bins = pa.BinSort(torch.empty(P, dtype=torch.int32, device=dev),
                                torch.empty(P, dtype=torch.int32, device=dev),
-                               torch.empty(nb + 1, dtype=torch.int32, device=dev),
-                               self.cell_lo, self.ncell, BRICK)
+                               torch.empty(nb + 1, dtype=torch.int32, device=dev), self.geom)
 
     def restore(self):
         self.c.copy_(self.c0)
 
+    def reorder(self):
+        """Store the particles in bin order (what a PIC code does when it
+        re-sorts its particle arrays); later sorts still run in full."""
+        self.pa.bin_sort(self.ctx, self.mesh, self.x, self.geom, out=self.bins)
+        perm = self.bins.perm.long()
+        for name in ("x", "m", "c", "c0"):
+            t = getattr(self, name)
+            setattr(self, name, t.t()[:, perm].contiguous().t())
+
     def run(self):
         pa, ctx = self.pa, self.ctx
-        pa.bin_sort(ctx, self.mesh, self.x, self.cell_lo, self.ncell, BRICK, out=self.bins)
+        pa.bin_sort(ctx, self.mesh, self.x, self.geom, out=self.bins)
         sd = pa.create_spline(pa.FieldLocation.Node, 2, self.mesh, self.x, self.bins)
         pa.APIC.p2g(ctx, self.m, self.c, self.gm, self.gmu, sd, zero_first=True)
         pa.grid_normalize(ctx, self.gm, self.gmu, self.gu,
@@ -153,10 +164,22 @@ def main():
     families = ["bin_sort", "zero_grid", "apic_p2g", "max_mass", "normalize", "apic_g2p"]
 
     s32 = Step(pa, ctx, wl, torch.float32, x32, m32, c32)
+    # One step on the shuffled storage order, for reference.
+    s32.run()
+    s32.restore()
+    torch.cuda.synchronize()
+    t_sh = time.perf_counter()
+    s32.run()
+    torch.cuda.synchronize()
+    shuffled_ms = (time.perf_counter() - t_sh) * 1e3
+    s32.restore()
+    log(f"fp32 step on shuffled storage: {shuffled_ms:.3f} ms")
+    s32.reorder()
     ms32, kt32 = time_steps(s32, args.steps, args.warmup, ctx, families)
     log(f"fp32: {ms32:.3f} ms/step, kernels {json.dumps(kt32)}")
 
     s64 = Step(pa, ctx, wl, torch.float64, wl.x, wl.m, wl.c)
+    s64.reorder()
     ms64, kt64 = time_steps(s64, args.steps, args.warmup, ctx, families)
     log(f"fp64: {ms64:.3f} ms/step, kernels {json.dumps(kt64)}")
     del s64
@@ -164,6 +187,7 @@ def main():
     # Error of the FP32 step against the FP64 step on the same rounded inputs.
     ref = Step(pa, ctx, wl, torch.float64, x32.astype(np.float64), m32.astype(np.float64),
                c32.astype(np.float64))
+    ref.reorder()
     ref.run()
     s32.restore()
     s32.run()
@@ -229,6 +253,7 @@ def main():
         "fp64": {"particles_per_s": P / (ms64 * 1e-3), "ms_per_step": ms64,
                  "kernels": kt64},
         "fp32_over_fp64": ms64 / ms32,
+        "fp32_step_ms_shuffled_storage": shuffled_ms,
         "fp32_error_vs_fp64": err,
         "step_alg_bytes": B_alg32,
         "step_alg_GBs": step_gbs,

@@ -104,16 +104,26 @@ typedef struct pic_pfield {
     int64_t cstride;
 } pic_pfield;
 
-/* Particles binned by pic_bin_sort: perm[s] is the particle in sorted slot
- * s; brick b holds slots [bin_start[b], bin_start[b+1]).  Cells (node
- * lattice, floor((x-low)/h)) [cell_lo, cell_lo+ncell) are grouped in bricks
- * of brick[3] cells. */
-typedef struct pic_bins {
-    const uint32_t* perm;
-    const uint32_t* bin_start;
+/* Binning geometry.  Bins are the cells of a lattice aligned with the mesh:
+ * bin index c = floor((x - low)/h + dual/2) per dimension, so dual = 0 bins
+ * by mesh cell and dual = 1 by node-centred (dual) cell -- the stencil base
+ * of quadratic splines on nodes.  Bins [cell_lo, cell_lo+ncell) are grouped
+ * in bricks of brick[3] bins; key = brick * brick_volume + bin_in_brick,
+ * both linearised i-major (k fastest). */
+typedef struct pic_binning {
     int32_t cell_lo[3];
     int32_t ncell[3];
     int32_t brick[3];
+    int32_t dual;
+} pic_binning;
+
+/* Particles binned by pic_bin_sort: perm[s] is the particle in sorted slot
+ * s, keys[s] its key; brick b holds slots [bin_start[b], bin_start[b+1]). */
+typedef struct pic_bins {
+    const uint32_t* perm;
+    const uint32_t* keys;
+    const uint32_t* bin_start;
+    pic_binning geom;
 } pic_bins;
 
 typedef struct pic_context pic_context;
@@ -192,17 +202,17 @@ int pic_update_F(pic_context* ctx, int64_t np, int gradu_dtype,
                  const pic_pfield* gradu, double dt, const pic_pfield* F,
                  const pic_pfield* det, const pic_pfield* Finv);
 
-/* Number of bricks for a binned region (the bin_start array needs one more
- * entry); returns a negative pic_status on invalid sizes. */
-int64_t pic_bin_count(const int32_t ncell[3], const int32_t brick[3]);
+/* Number of bricks of a binning (the bin_start array needs one more entry);
+ * a negative pic_status on invalid sizes. */
+int64_t pic_bin_count(const pic_binning* geom);
 
-/* Stable counting sort of particles by brick-major cell key (no reference
- * function; Picasso has no particle sort).  key = brick * brick_volume +
- * cell_in_brick, both i-major.  Writes keys[np], perm[np] and
- * bin_start[nbricks+1]; equal keys keep their input order. */
+/* Stable counting sort of particles by bin key (no reference function;
+ * Picasso has no particle sort).  Writes perm[np] (sorted slot -> particle),
+ * keys[np] (key of each sorted slot, non-decreasing) and
+ * bin_start[nbricks+1]; equal keys keep their input order.  A particle
+ * outside the bins is reported by pic_synchronize (outputs then undefined). */
 int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh,
-                 int64_t np, const pic_pfield* x, const int32_t cell_lo[3],
-                 const int32_t ncell[3], const int32_t brick[3],
+                 int64_t np, const pic_pfield* x, const pic_binning* geom,
                  uint32_t* keys, uint32_t* perm, uint32_t* bin_start);
 
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on

@@ -599,7 +599,8 @@ int orc_update_F(int64_t np, const double* gradu, double dt, double* F,
 
 int orc_cell_keys(int64_t np, const double* x, const double low[3], double h,
                   const int32_t cell_lo[3], const int32_t ncell[3],
-                  const int32_t brick[3], uint32_t* keys, int32_t* cells)
+                  const int32_t brick[3], int dual, uint32_t* keys,
+                  int32_t* cells)
 {
     if (!(h > 0.0)) return ORC_ERR_ARG;
     int64_t nb[3], bvol = 1, total = 1;
@@ -614,7 +615,7 @@ int orc_cell_keys(int64_t np, const double* x, const double low[3], double h,
     for (int64_t p = 0; p < np; ++p) {
         int64_t rel[3];
         for (int d = 0; d < 3; ++d) {
-            const double t = (x[3 * p + d] - low[d]) * rdx;
+            const double t = (x[3 * p + d] - low[d]) * rdx + (dual ? 0.5 : 0.0);
             const int64_t c = (int64_t)floor(t);
             if (cells) cells[3 * p + d] = (int32_t)c;
             rel[d] = c - cell_lo[d];

@@ -109,14 +109,16 @@ int orc_g2p_gradu(int order, int entity, const double low[3], double h,
 int orc_grid_normalize(int64_t nn, int ncomp, const double* m,
                        const double* mu, double* u);
 
-/* Cell keys.  cell = floor((x - low) * (1/h)) on the node lattice, relative
- * to cell_lo, must lie in [0, ncell).  Cells are grouped in bricks of
- * brick[3] cells; key = brick_linear * brick_volume + cell_in_brick_linear,
- * both linearisations i-major (k fastest).  brick = {1,1,1} gives the plain
+/* Cell keys.  cell = floor((x - low) * (1/h) + dual/2) (dual = 0: mesh
+ * cells, dual = 1: node-centred dual cells), relative to cell_lo, must lie
+ * in [0, ncell).  Cells are grouped in bricks of brick[3] cells;
+ * key = brick_linear * brick_volume + cell_in_brick_linear, both
+ * linearisations i-major (k fastest).  brick = {1,1,1} gives the plain
  * linear cell index.  cells[p][3] receives the absolute cell indices. */
 int orc_cell_keys(int64_t np, const double* x, const double low[3], double h,
                   const int32_t cell_lo[3], const int32_t ncell[3],
-                  const int32_t brick[3], uint32_t* keys, int32_t* cells);
+                  const int32_t brick[3], int dual, uint32_t* keys,
+                  int32_t* cells);
 
 /* Stable counting sort by key: perm[s] = index of the particle in sorted
  * slot s; key_start[k] = first slot of key k (nkeys+1 entries). */

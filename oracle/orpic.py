@@ -53,8 +53,8 @@ def _declare(L):
     L.orc_g2p_gradu.argtypes = [c_int, c_int, _D, c_double, _I32, _I32, c_i64,
                                 _D, _D, _D]
     L.orc_grid_normalize.argtypes = [c_i64, c_int, _D, _D, _D]
-    L.orc_cell_keys.argtypes = [c_i64, _D, _D, c_double, _I32, _I32, _I32, _U32,
-                                _I32]
+    L.orc_cell_keys.argtypes = [c_i64, _D, _D, c_double, _I32, _I32, _I32, c_int,
+                                _U32, _I32]
     L.orc_bin_sort.argtypes = [c_i64, _U32, ctypes.c_uint32, _U32, _U32]
     L.orc_det3.argtypes = [_D]
     L.orc_det3.restype = c_double
@@ -161,13 +161,13 @@ def grid_normalize(m, mu):
     return u
 
 
-def cell_keys(x, low, h, cell_lo, ncell, brick=(1, 1, 1)):
+def cell_keys(x, low, h, cell_lo, ncell, brick=(1, 1, 1), dual=0):
     x = _f64(x)
     keys = np.zeros(x.shape[0], np.uint32)
     cells = np.zeros((x.shape[0], 3), np.int32)
     _check(lib().orc_cell_keys(x.shape[0], _p(x), _p(_f64(low)), h,
                                _p(_i32(cell_lo), _I32), _p(_i32(ncell), _I32),
-                               _p(_i32(brick), _I32), _p(keys, _U32),
+                               _p(_i32(brick), _I32), int(dual), _p(keys, _U32),
                                _p(cells, _I32)), "cell_keys")
     return keys, cells
 

@@ -27,7 +27,8 @@ from ._abi import (CELL, EDGE_I, EDGE_J, EDGE_K, F32, F64, FACE_I, FACE_J, FACE_
                    PicError, check, lib)
 
 __all__ = ["Context", "Mesh", "GridField", "create_spline", "APIC", "PolyPIC", "FieldLocation",
-           "grid_normalize", "g2p_gradu", "update_F", "bin_sort", "BinSort", "PicError",
+           "grid_normalize", "g2p_gradu", "update_F", "bin_sort", "bin_count", "Binning",
+           "BinSort", "PicError",
            "lib", "LIB_PATH"]
 
 LIB_PATH = _abi.LIB_PATH
@@ -263,41 +264,52 @@ def update_F(ctx, gradu, dt, F, det, Finv):
 
 
 @dataclass
-class BinSort:
-    """Output of bin_sort: keys (input order), perm (sorted slot ->
-    particle), bin_start (per brick), and the binned region."""
-    keys: object
-    perm: object
-    bin_start: object
+class Binning:
+    """Bin lattice (pic_binning): bins floor((x-low)/h + dual/2) in
+    [cell_lo, cell_lo+ncell), grouped in bricks of `brick` bins."""
     cell_lo: Sequence[int]
     ncell: Sequence[int]
-    brick: Sequence[int]
+    brick: Sequence[int] = (8, 8, 8)
+    dual: int = 0
 
     def c(self):
         i3 = ctypes.c_int32 * 3
-        return _abi.Bins(self.perm.data_ptr(), self.bin_start.data_ptr(),
-                         i3(*self.cell_lo), i3(*self.ncell), i3(*self.brick))
+        return _abi.Binning(i3(*[int(v) for v in self.cell_lo]),
+                            i3(*[int(v) for v in self.ncell]),
+                            i3(*[int(v) for v in self.brick]), int(self.dual))
 
 
-def bin_count(ncell, brick):
-    i3 = ctypes.c_int32 * 3
-    return check(lib.pic_bin_count(i3(*ncell), i3(*brick)), "pic_bin_count")
+@dataclass
+class BinSort:
+    """Output of bin_sort: perm (sorted slot -> particle), keys (key of each
+    sorted slot), bin_start (per brick) and the binning."""
+    perm: object
+    keys: object
+    bin_start: object
+    geom: Binning
+
+    def c(self):
+        return _abi.Bins(self.perm.data_ptr(), self.keys.data_ptr(), self.bin_start.data_ptr(),
+                         self.geom.c())
 
 
-def bin_sort(ctx, mesh, x, cell_lo, ncell, brick, out=None):
-    """Stable counting sort of particles by brick-major cell key."""
+def bin_count(geom):
+    return check(lib.pic_bin_count(ctypes.byref(geom.c())), "pic_bin_count")
+
+
+def bin_sort(ctx, mesh, x, geom, out=None):
+    """Stable counting sort of particles by bin key (brick-major)."""
     import torch
-    nb = bin_count(ncell, brick)
+    nb = bin_count(geom)
     np_ = x.shape[0]
     if out is None:
-        keys = torch.empty(np_, dtype=torch.int32, device=x.device)
-        perm = torch.empty(np_, dtype=torch.int32, device=x.device)
-        start = torch.empty(nb + 1, dtype=torch.int32, device=x.device)
-        out = BinSort(keys, perm, start, tuple(cell_lo), tuple(ncell), tuple(brick))
-    i3 = ctypes.c_int32 * 3
+        out = BinSort(torch.empty(np_, dtype=torch.int32, device=x.device),
+                      torch.empty(np_, dtype=torch.int32, device=x.device),
+                      torch.empty(nb + 1, dtype=torch.int32, device=x.device), geom)
     m = mesh.c()
+    g = geom.c()
     rc = lib.pic_bin_sort(ctx._bind_stream(), _dtype_code(x), ctypes.byref(m), np_,
-                          ctypes.byref(_pfield(x, 3, "x")), i3(*cell_lo), i3(*ncell), i3(*brick),
+                          ctypes.byref(_pfield(x, 3, "x")), ctypes.byref(g),
                           out.keys.data_ptr(), out.perm.data_ptr(), out.bin_start.data_ptr())
     check(rc, "pic_bin_sort")
     return out

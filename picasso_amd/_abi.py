@@ -39,10 +39,14 @@ class PField(ctypes.Structure):
                 ("tile_stride", ctypes.c_int64), ("cstride", ctypes.c_int64)]
 
 
+class Binning(ctypes.Structure):
+    _fields_ = [("cell_lo", ctypes.c_int32 * 3), ("ncell", ctypes.c_int32 * 3),
+                ("brick", ctypes.c_int32 * 3), ("dual", ctypes.c_int32)]
+
+
 class Bins(ctypes.Structure):
-    _fields_ = [("perm", ctypes.c_void_p), ("bin_start", ctypes.c_void_p),
-                ("cell_lo", ctypes.c_int32 * 3), ("ncell", ctypes.c_int32 * 3),
-                ("brick", ctypes.c_int32 * 3)]
+    _fields_ = [("perm", ctypes.c_void_p), ("keys", ctypes.c_void_p),
+                ("bin_start", ctypes.c_void_p), ("geom", Binning)]
 
 
 _P = ctypes.POINTER
@@ -74,9 +78,9 @@ PROTOTYPES = {
                                _P(Grid), _P(PField), _P(Bins)]),
     "pic_update_F": (_c_int, [_vp, _c_i64, _c_int, _P(PField), _c_double, _P(PField),
                               _P(PField), _P(PField)]),
-    "pic_bin_count": (_c_i64, [_I32x3, _I32x3]),
-    "pic_bin_sort": (_c_int, [_vp, _c_int, _P(Mesh), _c_i64, _P(PField), _I32x3, _I32x3,
-                              _I32x3, _vp, _vp, _vp]),
+    "pic_bin_count": (_c_i64, [_P(Binning)]),
+    "pic_bin_sort": (_c_int, [_vp, _c_int, _P(Mesh), _c_i64, _P(PField), _P(Binning), _vp,
+                              _vp, _vp]),
     "pic_set_timing": (_c_int, [_vp, _c_int]),
     "pic_kernel_time": (_c_int, [_vp, ctypes.c_char_p, _P(_c_double), _P(_c_i64)]),
     "pic_reset_timing": (_c_int, [_vp]),

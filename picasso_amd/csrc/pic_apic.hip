@@ -29,6 +29,9 @@ __global__ __launch_bounds__(256) void k_apic_p2g_scatter(ApicArgs<T> a)
         return;
     }
     apic_p2g_particle<ORDER, STAG>(a, pt, sd, [&](int i, int j, int k, const T* mu, T m) {
+        i += sd.s0[0];
+        j += sd.s0[1];
+        k += sd.s0[2];
         const int64_t off = a.gmu.off(i, j, k);
         if constexpr (STAG) {
             atomic_add(a.gmu.data + off, mu[0]);
@@ -58,7 +61,7 @@ __global__ __launch_bounds__(256) void k_apic_g2p_gather(ApicArgs<T> a)
     }
     T cp[12];
     apic_g2p_particle<ORDER, STAG>(a, sd, [&](int i, int j, int k, T* ue) {
-        const int64_t off = a.gmu.off(i, j, k);
+        const int64_t off = a.gmu.off(sd.s0[0] + i, sd.s0[1] + j, sd.s0[2] + k);
         if constexpr (STAG) {
             ue[0] = a.gmu.data[off];
         } else {
@@ -73,26 +76,6 @@ __global__ __launch_bounds__(256) void k_apic_g2p_gather(ApicArgs<T> a)
 
 // ---------------------------------------------------------------------------
 // Host dispatch.
-
-template <typename T>
-static ApicArgs<T> make_args(pic_context* ctx, int entity, const pic_mesh& mesh,
-                             int64_t np, const pic_pfield* x, const pic_pfield* m,
-                             const pic_pfield* c, const pic_grid* gm, const pic_grid& gmu,
-                             const uint32_t* perm)
-{
-    ApicArgs<T> a;
-    a.lat = make_lattice(mesh, entity);
-    a.np = np;
-    a.perm = perm;
-    a.x = make_pfield<T>(*x);
-    a.m = m ? make_pfield<T>(*m) : PField<T>{};
-    a.c = make_pfield<T>(*c);
-    a.gm = gm ? make_gview<T>(*gm) : GView<T>{};
-    a.gmu = make_gview<T>(gmu);
-    a.sdim = stag_dim(entity) < 0 ? 0 : stag_dim(entity);
-    a.err = ctx->d_err;
-    return a;
-}
 
 template <typename T>
 struct P2GScatterLaunch {

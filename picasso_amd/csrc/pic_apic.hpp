@@ -86,7 +86,7 @@ __device__ __forceinline__ void apic_p2g_particle(const ApicArgs<T>& a,
                     for (int d = 0; d < NC; ++d)
                         mu[d] = wm * (pt.c0[d] + (DB[d][0] * sd.d[0][i] + DB[d][1] * sd.d[1][j] +
                                                   DB[d][2] * sd.d[2][k]));
-                    emit(sd.s0[0] + i, sd.s0[1] + j, sd.s0[2] + k, mu, wm);
+                    emit(i, j, k, mu, wm);
                 }
             }
     } else {
@@ -106,7 +106,7 @@ __device__ __forceinline__ void apic_p2g_particle(const ApicArgs<T>& a,
                     for (int d = 0; d < NC; ++d)
                         mu[d] = wm * pt.c0[d] +
                                 (pt.B[d][0] * gm[0] + pt.B[d][1] * gm[1] + pt.B[d][2] * gm[2]);
-                    emit(sd.s0[0] + i, sd.s0[1] + j, sd.s0[2] + k, mu, wm);
+                    emit(i, j, k, mu, wm);
                 }
     }
 }
@@ -137,7 +137,7 @@ __device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
             for (int k = 0; k < NK; ++k) {
                 const T wip = wij * sd.w[2][k];
                 T ue[NC];
-                fetch(sd.s0[0] + i, sd.s0[1] + j, sd.s0[2] + k, ue);
+                fetch(i, j, k, ue);
 #pragma unroll
                 for (int d = 0; d < NC; ++d) {
                     const T wu = wip * ue[d];
@@ -164,6 +164,26 @@ __device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
             for (int q = 0; q < 3; ++q) cp[(1 + q) * 3 + d] = B[d][q];
         }
     }
+}
+
+template <typename T>
+inline ApicArgs<T> make_args(pic_context* ctx, int entity, const pic_mesh& mesh,
+                             int64_t np, const pic_pfield* x, const pic_pfield* m,
+                             const pic_pfield* c, const pic_grid* gm, const pic_grid& gmu,
+                             const uint32_t* perm)
+{
+    ApicArgs<T> a;
+    a.lat = make_lattice(mesh, entity);
+    a.np = np;
+    a.perm = perm;
+    a.x = make_pfield<T>(*x);
+    a.m = m ? make_pfield<T>(*m) : PField<T>{};
+    a.c = make_pfield<T>(*c);
+    a.gm = gm ? make_gview<T>(*gm) : GView<T>{};
+    a.gmu = make_gview<T>(gmu);
+    a.sdim = stag_dim(entity) < 0 ? 0 : stag_dim(entity);
+    a.err = ctx->d_err;
+    return a;
 }
 
 // Order/staggering dispatch to a functor with a templated call operator.

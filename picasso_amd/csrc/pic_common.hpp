@@ -196,14 +196,25 @@ __device__ __forceinline__ void bspline_g2(T f, T* g2)
 // FP64 for both precisions: at 256 cells per unit length an FP32 position has
 // only ~16 bits below the cell size, and FP64 keeps the cell index bit-exact
 // with the oracle and the fraction accurate to FP32 rounding.
+// Contraction is off so the products and sums round exactly as in the oracle
+// and in the bin keys: cell indices are then bit-identical everywhere.
 template <int ORDER>
 __device__ __forceinline__ double knot_base(double x, double o, double rdx,
                                             double& f)
 {
+#pragma clang fp contract(off)
     const double t = (x - o) * rdx;
     const double b = ORDER == 2 ? floor(t + 0.5) : floor(t);
     f = t - b;
     return b;
+}
+
+// Bin coordinate floor((x - low)/h + dual/2) as a double (see pic_binning).
+__device__ __forceinline__ double bin_coord(double x, double low, double rdx, int dual)
+{
+#pragma clang fp contract(off)
+    const double t = (x - low) * rdx;
+    return floor(dual ? t + 0.5 : t);
 }
 
 template <int ORDER, typename T>

@@ -1,20 +1,25 @@
 // Cell-binning particle sort (no reference counterpart: Picasso never sorts
-// particles).  Stable counting sort by brick-major cell key:
-//   key = brick_linear * brick_volume + cell_in_brick_linear (both i-major),
-// so a brick's particles are contiguous and, inside it, ordered by cell and
+// particles).  Stable counting sort by bin key:
+//   key = brick_linear * brick_volume + bin_in_brick_linear (both i-major),
+// so a brick's particles are contiguous and, inside it, ordered by bin and
 // then by input index.
 //
-// Passes: (1) keys + per-brick counts (atomics), (2) exclusive scan of the
-// counts into bin_start, (3) scatter of (cell_in_brick, index) pairs into the
-// brick segments (atomic cursors, so unordered inside a segment), (4) one
-// workgroup per brick sorts its segment in LDS by (cell_in_brick, index),
-// which restores stability bit-exactly whatever order pass 3 produced.
+// Passes
+//  (1) keys in input order + per-brick counts (wave-aggregated atomics);
+//  (2) exclusive scan of the counts into bin_start;
+//  (3) scatter of particle indices into their brick segments (wave-aggregated
+//      cursors: order inside a segment depends on scheduling);
+//  (4) one workgroup per brick: LDS counting sort by bin, then each bin's
+//      indices sorted ascending, which restores the stable order bit-exactly
+//      whatever order pass 3 produced.  Writes perm and the sorted keys.
 #include "pic_common.hpp"
 #include "pic_internal.hpp"
+#include "pic_sort.hpp"
 
 namespace pic {
 
-constexpr int SORT_LDS_CAP = 8192;  // particles per brick sorted in LDS
+constexpr int SORT_LDS_CAP = 8192;     // particles per brick placed in LDS
+constexpr int SORT_MAX_BVOL = 4096;    // bins per brick
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;
 constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
@@ -26,43 +31,40 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
         p[i] = v;
 }
 
-struct KeyGeom {
-    double low[3];
-    double rdx;
-    int32_t cell_lo[3];
-    int32_t ncell[3];
-    int32_t brick[3];
-    int32_t nb[3];
-    uint32_t bvol;
-};
+// Wave-aggregated atomicAdd of 1 per active lane on base[key]: lanes with
+// equal keys share one atomic.  Returns each lane's slot (old value + rank of
+// the lane among equal-key lanes, in lane order).
+__device__ __forceinline__ uint32_t wave_agg_inc(uint32_t* base, uint32_t key, bool active)
+{
+    const int lane = __lane_id();
+    uint64_t pending = __ballot(active);
+    uint32_t slot = 0;
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lkey = __shfl(key, leader);
+        const uint64_t same = __ballot(active && key == lkey) & pending;
+        uint32_t b = 0;
+        if (lane == leader) b = atomicAdd(base + lkey, (uint32_t)__popcll(same));
+        b = __shfl(b, leader);
+        if ((same >> lane) & 1ull) slot = b + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return slot;
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_cell_keys(int64_t np, PField<T> x, KeyGeom g,
                                                    uint32_t* keys, uint32_t* count, int* err)
 {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    const int64_t xb = x.base(p);
-    int32_t rel[3];
-    bool ok = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double t = ((double)x.at(xb, d) - g.low[d]) * g.rdx;
-        const double c = floor(t) - (double)g.cell_lo[d];
-        ok = ok && (c >= 0.0) && (c < (double)g.ncell[d]);
-        rel[d] = ok ? (int32_t)c : 0;
+    const bool live = p < np;
+    uint32_t key = 0xffffffffu, b = 0;
+    if (live) {
+        key = cell_key(g, x, p, &b);
+        keys[p] = key;
+        if (key == 0xffffffffu) atomicOr(err, ERRBIT_CELL);
     }
-    if (!ok) {
-        keys[p] = 0xffffffffu;
-        atomicOr(err, ERRBIT_CELL);
-        return;
-    }
-    const uint32_t bi = rel[0] / g.brick[0], bj = rel[1] / g.brick[1], bk = rel[2] / g.brick[2];
-    const uint32_t li = rel[0] % g.brick[0], lj = rel[1] % g.brick[1], lk = rel[2] % g.brick[2];
-    const uint32_t b = (bi * g.nb[1] + bj) * g.nb[2] + bk;
-    const uint32_t l = (li * g.brick[1] + lj) * g.brick[2] + lk;
-    keys[p] = b * g.bvol + l;
-    atomicAdd(count + b, 1u);
+    (void)wave_agg_inc(count, b, live && key != 0xffffffffu);
 }
 
 // ---------------------------------------------------------------------------
@@ -120,9 +122,8 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_add(uint32_t* out, int64_
         if (base + q < n) out[base + q] += add;
 }
 
-// Exclusive scan of n values (in and out may alias); tmp needs
-// scan_tmp_words(n) words.
-static int64_t scan_tmp_words(int64_t n)
+// Words of temporary storage exclusive_scan needs for n values.
+int64_t scan_tmp_words(int64_t n)
 {
     int64_t w = 0;
     while (n > SCAN_TILE) {
@@ -132,8 +133,7 @@ static int64_t scan_tmp_words(int64_t n)
     return w + 1;
 }
 
-static void exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int64_t n,
-                           uint32_t* tmp)
+void exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp)
 {
     const int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nt <= 1) {
@@ -149,111 +149,152 @@ static void exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, in
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bin_scatter(int64_t np, const uint32_t* keys,
                                                      uint32_t bvol, const uint32_t* start,
-                                                     uint32_t* cursor, uint64_t* tmp)
+                                                     uint32_t* cursor, uint32_t* tmp)
 {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (p >= np) return;
-    const uint32_t key = keys[p];
-    if (key == 0xffffffffu) return;
-    const uint32_t b = key / bvol;
-    const uint32_t slot = start[b] + atomicAdd(cursor + b, 1u);
-    tmp[slot] = (uint64_t(key - b * bvol) << 32) | uint64_t(p);
+    const bool live = p < np;
+    const uint32_t key = live ? keys[p] : 0xffffffffu;
+    const bool ok = live && key != 0xffffffffu;
+    const uint32_t b = ok ? key / bvol : 0u;
+    const uint32_t r = wave_agg_inc(cursor, b, ok);
+    if (ok) tmp[start[b] + r] = uint32_t(p);
 }
 
-// One workgroup per brick: sort its segment by (cell_in_brick, index).
-__global__ __launch_bounds__(256) void k_brick_sort(const uint32_t* start, const uint64_t* tmp,
-                                                    uint32_t* perm)
+// Sort one brick segment held in `items` (LDS or global scratch).
+__device__ void brick_segment_sort(uint32_t b, uint32_t beg, uint32_t n, uint32_t bvol,
+                                   const uint32_t* tmp, const uint32_t* keys_in,
+                                   uint32_t* items, uint32_t* cnt, uint32_t* cur,
+                                   uint32_t* perm, uint32_t* keys_out)
 {
-    extern __shared__ uint64_t s[];
+    const uint32_t kb = b * bvol;
+    for (uint32_t l = threadIdx.x; l < bvol; l += blockDim.x) cnt[l] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        atomicAdd(&cnt[keys_in[tmp[beg + i]] - kb], 1u);
+    __syncthreads();
+    // exclusive scan of cnt into cur (bvol <= SORT_MAX_BVOL; serial chunks)
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        uint32_t carry = 0;
+        for (uint32_t c0 = 0; c0 < bvol; c0 += 64) {
+            const uint32_t v = (c0 + lane < bvol) ? cnt[c0 + lane] : 0u;
+            uint32_t inc = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(inc, o);
+                if (lane >= o) inc += y;
+            }
+            if (c0 + lane < bvol) cur[c0 + lane] = carry + inc - v;
+            carry += __shfl(inc, 63);
+        }
+    }
+    __syncthreads();
+    // place (unordered within a bin), remembering each bin's start in cnt
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t p = tmp[beg + i];
+        const uint32_t pos = atomicAdd(&cur[keys_in[p] - kb], 1u);
+        items[pos] = p;
+    }
+    __syncthreads();
+    // each bin: ascending indices (insertion sort; bins hold a few particles)
+    for (uint32_t l = threadIdx.x; l < bvol; l += blockDim.x) {
+        const uint32_t e = cur[l], c = cnt[l], s0 = e - c;
+        for (uint32_t i = s0 + 1; i < e; ++i) {
+            const uint32_t v = items[i];
+            uint32_t j = i;
+            while (j > s0 && items[j - 1] > v) {
+                items[j] = items[j - 1];
+                --j;
+            }
+            items[j] = v;
+        }
+        for (uint32_t i = s0; i < e; ++i) keys_out[beg + i] = kb + l;
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) perm[beg + i] = items[i];
+}
+
+__global__ __launch_bounds__(256) void k_brick_sort(const uint32_t* start, const uint32_t* tmp,
+                                                    const uint32_t* keys_in, uint32_t bvol,
+                                                    uint32_t* big, uint32_t* perm,
+                                                    uint32_t* keys_out)
+{
+    __shared__ uint32_t cnt[SORT_MAX_BVOL];
+    __shared__ uint32_t cur[SORT_MAX_BVOL];
+    __shared__ uint32_t lds_items[SORT_LDS_CAP];
     const uint32_t b = blockIdx.x;
     const uint32_t beg = start[b], n = start[b + 1] - beg;
     if (n == 0) return;
-    if (n <= SORT_LDS_CAP) {
-        uint32_t N = 1;
-        while (N < n) N <<= 1;
-        for (uint32_t i = threadIdx.x; i < N; i += blockDim.x)
-            s[i] = i < n ? tmp[beg + i] : ~uint64_t(0);
-        __syncthreads();
-        for (uint32_t k = 2; k <= N; k <<= 1)
-            for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-                for (uint32_t i = threadIdx.x; i < N; i += blockDim.x) {
-                    const uint32_t ixj = i ^ j;
-                    if (ixj > i) {
-                        const uint64_t a = s[i], c = s[ixj];
-                        const bool up = (i & k) == 0;
-                        if ((a > c) == up) {
-                            s[i] = c;
-                            s[ixj] = a;
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) perm[beg + i] = uint32_t(s[i]);
-    } else {
-        // Oversized brick: rank by counting (keys are unique).
-        for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-            const uint64_t v = tmp[beg + i];
-            uint32_t r = 0;
-            for (uint32_t q = 0; q < n; ++q) r += tmp[beg + q] < v;
-            perm[beg + r] = uint32_t(v);
-        }
-    }
+    uint32_t* items = n <= SORT_LDS_CAP ? lds_items : big + beg;
+    brick_segment_sort(b, beg, n, bvol, tmp, keys_in, items, cnt, cur, perm, keys_out);
 }
 
 }  // namespace pic
 
 using namespace pic;
 
-extern "C" int64_t pic_bin_count(const int32_t ncell[3], const int32_t brick[3])
+extern "C" int64_t pic_bin_count(const pic_binning* g)
 {
-    if (!ncell || !brick) return PIC_ERR_ARG;
+    if (!g) return PIC_ERR_ARG;
     int64_t nb = 1, bvol = 1;
     for (int d = 0; d < 3; ++d) {
-        if (ncell[d] <= 0 || brick[d] <= 0) return PIC_ERR_ARG;
-        nb *= (ncell[d] + brick[d] - 1) / brick[d];
-        bvol *= brick[d];
+        if (g->ncell[d] <= 0 || g->brick[d] <= 0) return PIC_ERR_ARG;
+        nb *= (g->ncell[d] + g->brick[d] - 1) / g->brick[d];
+        bvol *= g->brick[d];
     }
-    if (nb * bvol >= int64_t(UINT32_MAX) || nb >= int64_t(INT32_MAX)) return PIC_ERR_CAPACITY;
+    if (g->dual != 0 && g->dual != 1) return PIC_ERR_ARG;
+    if (bvol > SORT_MAX_BVOL || nb * bvol >= int64_t(UINT32_MAX) || nb >= int64_t(INT32_MAX))
+        return PIC_ERR_CAPACITY;
     return nb;
 }
 
-extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, int64_t np,
-                            const pic_pfield* x, const int32_t cell_lo[3], const int32_t ncell[3],
-                            const int32_t brick[3], uint32_t* keys, uint32_t* perm,
-                            uint32_t* bin_start)
+namespace pic {
+
+KeyGeom make_key_geom(const pic_mesh& mesh, const pic_binning& b)
 {
-    if (!ctx || !mesh || !(mesh->h > 0.0) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
-        np >= int64_t(UINT32_MAX) || !cell_lo || !ncell || !brick || !bin_start)
-        return PIC_ERR_ARG;
-    if (np > 0 && (!valid_pfield(x) || !keys || !perm)) return PIC_ERR_ARG;
-    const int64_t nb = pic_bin_count(ncell, brick);
-    if (nb < 0) return (int)nb;
     KeyGeom g;
     g.bvol = 1;
     for (int d = 0; d < 3; ++d) {
-        g.low[d] = mesh->low[d];
-        g.cell_lo[d] = cell_lo[d];
-        g.ncell[d] = ncell[d];
-        g.brick[d] = brick[d];
-        g.nb[d] = (ncell[d] + brick[d] - 1) / brick[d];
-        g.bvol *= brick[d];
+        g.low[d] = mesh.low[d];
+        g.cell_lo[d] = b.cell_lo[d];
+        g.ncell[d] = b.ncell[d];
+        g.brick[d] = b.brick[d];
+        g.nb[d] = (b.ncell[d] + b.brick[d] - 1) / b.brick[d];
+        g.bvol *= b.brick[d];
     }
-    g.rdx = 1.0 / mesh->h;
+    g.rdx = 1.0 / mesh.h;
+    g.dual = b.dual;
+    return g;
+}
+
+}  // namespace pic
+
+extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, int64_t np,
+                            const pic_pfield* x, const pic_binning* geom, uint32_t* keys,
+                            uint32_t* perm, uint32_t* bin_start)
+{
+    if (!ctx || !mesh || !(mesh->h > 0.0) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
+        np >= int64_t(UINT32_MAX) || !geom || !bin_start)
+        return PIC_ERR_ARG;
+    if (np > 0 && (!valid_pfield(x) || !keys || !perm)) return PIC_ERR_ARG;
+    const int64_t nb = pic_bin_count(geom);
+    if (nb < 0) return (int)nb;
+    const KeyGeom g = make_key_geom(*mesh, *geom);
     DeviceGuard dg(ctx->device);
     if (!dg.ok) return PIC_ERR_HIP;
 
-    // scratch: count[nb+1] | cursor[nb] | scan tmp | tmp pairs (8-aligned)
+    // scratch: count[nb+1] | cursor[nb] | scan tmp | keys_in[np] | tmp[np] | big[np]
     const int64_t scan_words = scan_tmp_words(nb + 1);
-    int64_t words = (nb + 1) + nb + scan_words;
-    words = (words + 1) & ~int64_t(1);
-    const size_t bytes = size_t(words) * 4 + size_t(np) * 8;
+    const int64_t head = (nb + 1) + nb + scan_words;
+    const size_t bytes = size_t(head + 3 * np) * 4;
     uint32_t* sc = static_cast<uint32_t*>(scratch(ctx, bytes));
     if (!sc) return PIC_ERR_ALLOC;
     uint32_t* count = sc;
     uint32_t* cursor = count + nb + 1;
     uint32_t* scan_tmp = cursor + nb;
-    uint64_t* tmp = reinterpret_cast<uint64_t*>(sc + words);
+    uint32_t* keys_in = sc + head;
+    uint32_t* tmp = keys_in + np;
+    uint32_t* big = tmp + np;
     hipStream_t st = ctx->stream;
 
     TimedLaunch tl(ctx, "bin_sort");
@@ -262,18 +303,18 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
     if (np > 0) {
         const unsigned b = blocks_for(np, 256);
         if (dtype == PIC_F64)
-            k_cell_keys<double><<<b, 256, 0, st>>>(np, make_pfield<double>(*x), g, keys, count,
+            k_cell_keys<double><<<b, 256, 0, st>>>(np, make_pfield<double>(*x), g, keys_in, count,
                                                     ctx->d_err);
         else
-            k_cell_keys<float><<<b, 256, 0, st>>>(np, make_pfield<float>(*x), g, keys, count,
+            k_cell_keys<float><<<b, 256, 0, st>>>(np, make_pfield<float>(*x), g, keys_in, count,
                                                    ctx->d_err);
     }
     exclusive_scan(st, count, bin_start, nb + 1, scan_tmp);
     if (np > 0) {
-        k_bin_scatter<<<blocks_for(np, 256), 256, 0, st>>>(np, keys, g.bvol, bin_start, cursor,
+        k_bin_scatter<<<blocks_for(np, 256), 256, 0, st>>>(np, keys_in, g.bvol, bin_start, cursor,
                                                             tmp);
-        k_brick_sort<<<(unsigned)nb, 256, SORT_LDS_CAP * sizeof(uint64_t), st>>>(bin_start, tmp,
-                                                                               perm);
+        k_brick_sort<<<(unsigned)nb, 256, 0, st>>>(bin_start, tmp, keys_in, g.bvol, big, perm,
+                                                   keys);
     }
     return launch_status();
 }

@@ -42,9 +42,17 @@ def test_host_only_entry_points():
     assert lib.pic_polypic_nmode(3) == 27
     assert lib.pic_polypic_nmode(4) == _abi.PIC_ERR_ARG
     i3 = ctypes.c_int32 * 3
-    assert lib.pic_bin_count(i3(256, 256, 256), i3(8, 8, 8)) == 32 ** 3
-    assert lib.pic_bin_count(i3(10, 10, 10), i3(4, 4, 4)) == 27
-    assert lib.pic_bin_count(i3(0, 10, 10), i3(4, 4, 4)) == _abi.PIC_ERR_ARG
+
+    def count(ncell, brick, dual=0):
+        g = _abi.Binning(i3(0, 0, 0), i3(*ncell), i3(*brick), dual)
+        return lib.pic_bin_count(ctypes.byref(g))
+
+    assert count((256, 256, 256), (8, 8, 8)) == 32 ** 3
+    assert count((257, 257, 257), (8, 8, 8), 1) == 33 ** 3
+    assert count((10, 10, 10), (4, 4, 4)) == 27
+    assert count((0, 10, 10), (4, 4, 4)) == _abi.PIC_ERR_ARG
+    assert count((10, 10, 10), (4, 4, 4), 2) == _abi.PIC_ERR_ARG
+    assert count((64, 64, 64), (32, 32, 8)) == _abi.PIC_ERR_CAPACITY
 
 
 def test_null_context_is_an_argument_error():

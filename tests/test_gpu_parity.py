@@ -144,26 +144,132 @@ def test_apic_entities_random(pa, entity, order):
     assert gu.rel_inf(gu.host(cg).reshape(-1, 4, 3), rc) < F64_TOL
 
 
+def _binning(pa, wl, order=2, entity=0, brick=(8, 8, 8)):
+    """Bins aligned with the spline's knot base for (order, entity)."""
+    dual = 1 if (order == 2) == (entity == 0) else 0
+    cell_lo, ncell = wl.cell_region
+    return pa.Binning(tuple(int(v) for v in cell_lo), tuple(int(v) + dual for v in ncell), brick,
+                      dual)
+
+
 @pytest.mark.parametrize("brick", [(1, 1, 1), (4, 4, 4), (8, 8, 8), (2, 4, 8)])
 @pytest.mark.parametrize("dt", ["f64", "f32"])
-def test_bin_sort_bit_exact(pa, wl0, brick, dt):
+@pytest.mark.parametrize("dual", [0, 1])
+def test_bin_sort_bit_exact(pa, wl0, brick, dt, dual):
     import gpu_util as gu
     tdt = torch.float64 if dt == "f64" else torch.float32
     xn = wl0.x if dt == "f64" else wl0.x.astype(np.float32)
     x = gu.soa(xn, tdt)
     cell_lo, ncell = wl0.cell_region
-    res = pa.bin_sort(gu.ctx(), pa.Mesh(wl0.low, wl0.h), x, tuple(cell_lo), tuple(ncell), brick)
+    geom = pa.Binning(tuple(int(v) for v in cell_lo), tuple(int(v) + dual for v in ncell), brick,
+                      dual)
+    res = pa.bin_sort(gu.ctx(), pa.Mesh(wl0.low, wl0.h), x, geom)
     gu.ctx().synchronize()
-    keys, _ = orpic.cell_keys(xn.astype(np.float64), wl0.low, wl0.h, cell_lo, ncell, brick)
-    nb = pa.bin_count(tuple(ncell), brick)
+    keys, _ = orpic.cell_keys(xn.astype(np.float64), wl0.low, wl0.h, geom.cell_lo, geom.ncell,
+                              brick, dual)
+    nb = pa.bin_count(geom)
     bvol = int(np.prod(brick))
     perm, kstart = orpic.bin_sort(keys, nb * bvol)
-    np.testing.assert_array_equal(gu.host(res.keys).view(np.uint32), keys)
     np.testing.assert_array_equal(gu.host(res.perm).view(np.uint32), perm)
+    np.testing.assert_array_equal(gu.host(res.keys).view(np.uint32), keys[perm])
     np.testing.assert_array_equal(gu.host(res.bin_start).view(np.uint32), kstart[::bvol])
 
 
-def test_binned_p2g_matches_unbinned(pa, wl0):
+def test_bin_sort_out_of_range_reported(pa, wl0):
+    import gpu_util as gu
+    x = gu.soa(wl0.x)
+    geom = pa.Binning((3, 2, 2), (31, 32, 32), (8, 8, 8), 0)  # cells at i = 2 left out
+    pa.bin_sort(gu.ctx(), pa.Mesh(wl0.low, wl0.h), x, geom)
+    with pytest.raises(pa.PicError):
+        gu.ctx().synchronize()
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("entity", [0, 1])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_brick_transfers_vs_oracle(pa, wl0, order, entity, dtype):
+    """Binned p2g/g2p (brick kernels) against the oracle."""
+    import gpu_util as gu
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    f32 = dtype == "f32"
+    xh = wl0.x.astype(np.float32).astype(np.float64) if f32 else wl0.x
+    mh = wl0.m.astype(np.float32).astype(np.float64) if f32 else wl0.m
+    ch = wl0.c.astype(np.float32).astype(np.float64) if f32 else wl0.c
+    lo, n = wl0.node_window
+    mesh = pa.Mesh(wl0.low, wl0.h)
+    x = gu.soa(xh, tdt)
+    m = gu.soa(mh[:, None], tdt)
+    c = gu.soa(ch.reshape(-1, 12), tdt)
+    bins = pa.bin_sort(gu.ctx(), mesh, x, _binning(pa, wl0, order, entity))
+    sd = pa.create_spline(entity, order, mesh, x, bins)
+    gm, gmu = _node_grids(pa, lo, n, tdt)
+    pa.APIC.p2g(gu.ctx(), m, c, gm, gmu, sd, zero_first=True)
+    rng = np.random.default_rng(order)
+    gfield = rng.normal(size=tuple(n) + (3,))
+    if f32:
+        gfield = gfield.astype(np.float32).astype(np.float64)
+    cg = torch.zeros_like(c)
+    pa.APIC.g2p(gu.ctx(), pa.GridField(gu.dev(gfield, tdt)), cg, sd)
+    gu.ctx().synchronize()
+    rm = np.zeros(tuple(n))
+    rmu = np.zeros(tuple(n) + (3,))
+    orpic.apic_p2g(order, entity, wl0.low, wl0.h, lo, n, xh, mh, ch, rm, rmu)
+    rc = orpic.apic_g2p(order, entity, wl0.low, wl0.h, lo, n, xh, gfield)
+    tol = F32_TOL if f32 else F64_TOL
+    assert gu.rel_inf(gu.host(gm.tensor), rm) < tol
+    assert gu.rel_inf(gu.host(gmu.tensor), rmu) < tol
+    assert gu.rel_inf(gu.host(cg).reshape(-1, 4, 3)[:, 0], rc[:, 0]) < tol
+    assert gu.rel_inf(gu.host(cg).reshape(-1, 4, 3)[:, 1:], rc[:, 1:]) < tol * 10
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_brick_with_stale_bins(pa, wl0, dtype):
+    """Bins computed before a third of the particles moved: the brick
+    kernels route those particles through the slow path, results still
+    match the oracle on the new positions."""
+    import gpu_util as gu
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    f32 = dtype == "f32"
+    lo, n = wl0.node_window
+    mesh = pa.Mesh(wl0.low, wl0.h)
+    xh = wl0.x.astype(np.float32).astype(np.float64) if f32 else wl0.x.copy()
+    x = gu.soa(xh, tdt)
+    bins = pa.bin_sort(gu.ctx(), mesh, x, _binning(pa, wl0))
+    gu.ctx().synchronize()
+    rng = np.random.default_rng(9)
+    moved = xh.copy()
+    sel = rng.random(len(xh)) < 0.33
+    moved[sel] += rng.uniform(-1.5, 1.5, size=(sel.sum(), 3)) * wl0.h
+    moved = np.clip(moved, 0.0, 1.0 - 1e-9)
+    if f32:
+        moved = moved.astype(np.float32).astype(np.float64)
+    x.copy_(gu.soa(moved, tdt))
+    mh = wl0.m.astype(np.float32).astype(np.float64) if f32 else wl0.m
+    ch = wl0.c.astype(np.float32).astype(np.float64) if f32 else wl0.c
+    m = gu.soa(mh[:, None], tdt)
+    c = gu.soa(ch.reshape(-1, 12), tdt)
+    sd = pa.create_spline(0, 2, mesh, x, bins)
+    gm, gmu = _node_grids(pa, lo, n, tdt)
+    pa.APIC.p2g(gu.ctx(), m, c, gm, gmu, sd, zero_first=True)
+    gfield = np.random.default_rng(3).normal(size=tuple(n) + (3,))
+    if f32:
+        gfield = gfield.astype(np.float32).astype(np.float64)
+    cg = torch.zeros_like(c)
+    pa.APIC.g2p(gu.ctx(), pa.GridField(gu.dev(gfield, tdt)), cg, sd)
+    gu.ctx().synchronize()
+    rm = np.zeros(tuple(n))
+    rmu = np.zeros(tuple(n) + (3,))
+    orpic.apic_p2g(2, 0, wl0.low, wl0.h, lo, n, moved, mh, ch, rm, rmu)
+    rc = orpic.apic_g2p(2, 0, wl0.low, wl0.h, lo, n, moved, gfield)
+    tol = F32_TOL if f32 else F64_TOL
+    assert gu.rel_inf(gu.host(gm.tensor), rm) < tol
+    assert gu.rel_inf(gu.host(gmu.tensor), rmu) < tol
+    assert gu.rel_inf(gu.host(cg).reshape(-1, 4, 3), rc) < tol * 10
+
+
+def test_unaligned_bins_fall_back(pa, wl0):
+    """Mesh-cell bins with quadratic node splines are not knot-aligned: the
+    transfers fall back to the scatter/gather kernels and stay exact."""
     import gpu_util as gu
     lo, n = wl0.node_window
     x = gu.soa(wl0.x)
@@ -171,20 +277,14 @@ def test_binned_p2g_matches_unbinned(pa, wl0):
     c = gu.soa(wl0.c.reshape(-1, 12))
     mesh = pa.Mesh(wl0.low, wl0.h)
     cell_lo, ncell = wl0.cell_region
-    bins = pa.bin_sort(gu.ctx(), mesh, x, tuple(cell_lo), tuple(ncell), (8, 8, 8))
+    bins = pa.bin_sort(gu.ctx(), mesh, x, pa.Binning(tuple(cell_lo), tuple(ncell), (8, 8, 8), 0))
     gm, gmu = _node_grids(pa, lo, n, torch.float64)
     pa.APIC.p2g(gu.ctx(), m, c, gm, gmu, pa.create_spline(0, 2, mesh, x, bins))
-    gm2, gmu2 = _node_grids(pa, lo, n, torch.float64)
-    pa.APIC.p2g(gu.ctx(), m, c, gm2, gmu2, pa.create_spline(0, 2, mesh, x))
-    c_b = torch.zeros_like(c)
-    c_u = torch.zeros_like(c)
-    u = pa.GridField(torch.randn_like(gmu.tensor), gmu.lo)
-    pa.APIC.g2p(gu.ctx(), u, c_b, pa.create_spline(0, 2, mesh, x, bins))
-    pa.APIC.g2p(gu.ctx(), u, c_u, pa.create_spline(0, 2, mesh, x))
     gu.ctx().synchronize()
-    assert gu.rel_inf(gu.host(gm.tensor), gu.host(gm2.tensor)) < F64_TOL
-    assert gu.rel_inf(gu.host(gmu.tensor), gu.host(gmu2.tensor)) < F64_TOL
-    assert gu.rel_inf(gu.host(c_b), gu.host(c_u)) < F64_TOL
+    rm = np.zeros(tuple(n))
+    rmu = np.zeros(tuple(n) + (3,))
+    orpic.apic_p2g(2, 0, wl0.low, wl0.h, lo, n, wl0.x, wl0.m, wl0.c, rm, rmu)
+    assert gu.rel_inf(gu.host(gmu.tensor), rmu) < F64_TOL
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
@@ -285,6 +385,6 @@ def test_empty_inputs(pa):
                 pa.create_spline(0, 2, mesh, x), zero_first=True)
     gu.ctx().synchronize()
     assert gu.host(gm.tensor).sum() == 0.0 and gu.host(gmu.tensor).sum() == 0.0
-    res = pa.bin_sort(gu.ctx(), mesh, x, (0, 0, 0), (4, 4, 4), (2, 2, 2))
+    res = pa.bin_sort(gu.ctx(), mesh, x, pa.Binning((0, 0, 0), (4, 4, 4), (2, 2, 2), 0))
     gu.ctx().synchronize()
     assert np.all(gu.host(res.bin_start) == 0)
