@@ -63,9 +63,10 @@ class Step:
         self.geom = pa.Binning(tuple(int(v) for v in cell_lo), tuple(int(v) + 1 for v in ncell),
                                BRICK, 1)
         nb = pa.bin_count(self.geom)
-        self.bins = pa.BinSort(torch.empty(P, dtype=torch.int32, device=dev),
-                               torch.empty(P, dtype=torch.int32, device=dev),
-                               torch.empty(nb + 1, dtype=torch.int32, device=dev), self.geom)
+        i32 = dict(dtype=torch.int32, device=dev)
+        self.bins = pa.BinSort(torch.empty(P, **i32), torch.empty(P, **i32),
+                               torch.empty(nb + 1, **i32), self.geom, torch.empty(nb + 1, **i32))
+        self.fused = True
 
     def restore(self):
         self.c.copy_(self.c0)
@@ -84,9 +85,13 @@ class Step:
         pa.bin_sort(ctx, self.mesh, self.x, self.geom, out=self.bins)
         sd = pa.create_spline(pa.FieldLocation.Node, 2, self.mesh, self.x, self.bins)
         pa.APIC.p2g(ctx, self.m, self.c, self.gm, self.gmu, sd, zero_first=True)
-        pa.grid_normalize(ctx, self.gm, self.gmu, self.gu,
-                          rel_mass_eps=1e-6 if self.dtype is self.torch.float32 else 0.0)
-        pa.APIC.g2p(ctx, self.gu, self.c, sd)
+        if self.fused:
+            # grid normalisation fused into g2p: u = mu/m formed in LDS
+            pa.APIC.g2p_normalized(ctx, self.gm, self.gmu, self.c, sd)
+        else:
+            pa.grid_normalize(ctx, self.gm, self.gmu, self.gu,
+                              rel_mass_eps=1e-6 if self.dtype is self.torch.float32 else 0.0)
+            pa.APIC.g2p(ctx, self.gu, self.c, sd)
 
 
 def time_steps(step, steps, warmup, ctx, families):

@@ -118,11 +118,14 @@ typedef struct pic_binning {
 } pic_binning;
 
 /* Particles binned by pic_bin_sort: perm[s] is the particle in sorted slot
- * s, keys[s] its key; brick b holds slots [bin_start[b], bin_start[b+1]). */
+ * s, keys[s] its key; brick b holds slots [bin_start[b], bin_start[b+1]).
+ * active (optional, may be NULL) lists the non-empty bricks: active[0] is
+ * their number, active[1..] their indices in increasing order. */
 typedef struct pic_bins {
     const uint32_t* perm;
     const uint32_t* keys;
     const uint32_t* bin_start;
+    const uint32_t* active;
     pic_binning geom;
 } pic_bins;
 
@@ -156,6 +159,17 @@ int pic_apic_g2p(pic_context* ctx, int order, int entity, int dtype,
                  const pic_mesh* mesh, int64_t np, const pic_pfield* x,
                  const pic_grid* u_i, const pic_pfield* c_p,
                  const pic_bins* bins);
+
+/* APIC::g2p of the grid velocity u_i = mu_i / m_i (0 where m_i == 0),
+ * computed on the fly from grid mass and momentum: grid normalisation
+ * (doc/basic/explict_nearly_incompressible_flow.md:18-23) fused into
+ * APIC::g2p (Picasso_APIC.hpp:407-544), without materialising u_i.  The
+ * division runs in FP64 for both dtypes.  m_i and mu_i share one window. */
+int pic_apic_g2p_normalized(pic_context* ctx, int order, int entity,
+                            int dtype, const pic_mesh* mesh, int64_t np,
+                            const pic_pfield* x, const pic_grid* m_i,
+                            const pic_grid* mu_i, const pic_pfield* c_p,
+                            const pic_bins* bins);
 
 /* Number of PolyPIC modes per component for a spline order: 8 (linear, the
  * reference's), 27 (orders 2 and 3, the {1,y,y^2}^3 basis). */
@@ -209,11 +223,14 @@ int64_t pic_bin_count(const pic_binning* geom);
 /* Stable counting sort of particles by bin key (no reference function;
  * Picasso has no particle sort).  Writes perm[np] (sorted slot -> particle),
  * keys[np] (key of each sorted slot, non-decreasing) and
- * bin_start[nbricks+1]; equal keys keep their input order.  A particle
- * outside the bins is reported by pic_synchronize (outputs then undefined). */
+ * bin_start[nbricks+1]; equal keys keep their input order.  When active is
+ * not NULL (nbricks+1 entries) it receives the list of non-empty bricks
+ * described at pic_bins.  A particle outside the bins is reported by
+ * pic_synchronize (outputs then undefined). */
 int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh,
                  int64_t np, const pic_pfield* x, const pic_binning* geom,
-                 uint32_t* keys, uint32_t* perm, uint32_t* bin_start);
+                 uint32_t* keys, uint32_t* perm, uint32_t* bin_start,
+                 uint32_t* active);
 
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on
  * the context's stream and pic_kernel_time() returns the accumulated

@@ -197,6 +197,20 @@ class APIC:
                               _bins_ptr(sd.bins))
         check(rc, "pic_apic_g2p")
 
+    @staticmethod
+    def g2p_normalized(ctx, m_i, mu_i, c_p, sd):
+        """APIC::g2p of u_i = mu_i / m_i computed on the fly (grid
+        normalisation fused into g2p)."""
+        x = sd.x
+        mesh = sd.mesh.c()
+        rc = lib.pic_apic_g2p_normalized(ctx._bind_stream(), sd.order, sd.location,
+                                         _dtype_code(x), ctypes.byref(mesh), x.shape[0],
+                                         ctypes.byref(_pfield(x, 3, "x")), ctypes.byref(m_i.c()),
+                                         ctypes.byref(mu_i.c()),
+                                         ctypes.byref(_pfield(c_p, 12, "c_p")),
+                                         _bins_ptr(sd.bins))
+        check(rc, "pic_apic_g2p_normalized")
+
 
 class PolyPIC:
     @staticmethod
@@ -282,14 +296,17 @@ class Binning:
 @dataclass
 class BinSort:
     """Output of bin_sort: perm (sorted slot -> particle), keys (key of each
-    sorted slot), bin_start (per brick) and the binning."""
+    sorted slot), bin_start (per brick), the list of non-empty bricks
+    (active, optional) and the binning."""
     perm: object
     keys: object
     bin_start: object
     geom: Binning
+    active: object = None
 
     def c(self):
         return _abi.Bins(self.perm.data_ptr(), self.keys.data_ptr(), self.bin_start.data_ptr(),
+                         self.active.data_ptr() if self.active is not None else None,
                          self.geom.c())
 
 
@@ -297,19 +314,21 @@ def bin_count(geom):
     return check(lib.pic_bin_count(ctypes.byref(geom.c())), "pic_bin_count")
 
 
-def bin_sort(ctx, mesh, x, geom, out=None):
+def bin_sort(ctx, mesh, x, geom, out=None, with_active=True):
     """Stable counting sort of particles by bin key (brick-major)."""
     import torch
     nb = bin_count(geom)
     np_ = x.shape[0]
     if out is None:
-        out = BinSort(torch.empty(np_, dtype=torch.int32, device=x.device),
-                      torch.empty(np_, dtype=torch.int32, device=x.device),
-                      torch.empty(nb + 1, dtype=torch.int32, device=x.device), geom)
+        i32 = dict(dtype=torch.int32, device=x.device)
+        out = BinSort(torch.empty(np_, **i32), torch.empty(np_, **i32),
+                      torch.empty(nb + 1, **i32), geom,
+                      torch.empty(nb + 1, **i32) if with_active else None)
     m = mesh.c()
     g = geom.c()
     rc = lib.pic_bin_sort(ctx._bind_stream(), _dtype_code(x), ctypes.byref(m), np_,
                           ctypes.byref(_pfield(x, 3, "x")), ctypes.byref(g),
-                          out.keys.data_ptr(), out.perm.data_ptr(), out.bin_start.data_ptr())
+                          out.keys.data_ptr(), out.perm.data_ptr(), out.bin_start.data_ptr(),
+                          out.active.data_ptr() if out.active is not None else None)
     check(rc, "pic_bin_sort")
     return out

@@ -46,7 +46,8 @@ class Binning(ctypes.Structure):
 
 class Bins(ctypes.Structure):
     _fields_ = [("perm", ctypes.c_void_p), ("keys", ctypes.c_void_p),
-                ("bin_start", ctypes.c_void_p), ("geom", Binning)]
+                ("bin_start", ctypes.c_void_p), ("active", ctypes.c_void_p),
+                ("geom", Binning)]
 
 
 _P = ctypes.POINTER
@@ -80,7 +81,9 @@ PROTOTYPES = {
                               _P(PField), _P(PField)]),
     "pic_bin_count": (_c_i64, [_P(Binning)]),
     "pic_bin_sort": (_c_int, [_vp, _c_int, _P(Mesh), _c_i64, _P(PField), _P(Binning), _vp,
-                              _vp, _vp]),
+                              _vp, _vp, _vp]),
+    "pic_apic_g2p_normalized": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(Mesh), _c_i64,
+                                         _P(PField), _P(Grid), _P(Grid), _P(PField), _P(Bins)]),
     "pic_set_timing": (_c_int, [_vp, _c_int]),
     "pic_kernel_time": (_c_int, [_vp, ctypes.c_char_p, _P(_c_double), _P(_c_i64)]),
     "pic_reset_timing": (_c_int, [_vp]),

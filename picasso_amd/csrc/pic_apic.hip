@@ -61,12 +61,14 @@ __global__ __launch_bounds__(256) void k_apic_g2p_gather(ApicArgs<T> a)
     }
     T cp[12];
     apic_g2p_particle<ORDER, STAG>(a, sd, [&](int i, int j, int k, T* ue) {
-        const int64_t off = a.gmu.off(sd.s0[0] + i, sd.s0[1] + j, sd.s0[2] + k);
+        const int ni = sd.s0[0] + i, nj = sd.s0[1] + j, nk = sd.s0[2] + k;
+        const int64_t off = a.gmu.off(ni, nj, nk);
+        const int64_t moff = a.gnm.data ? a.gnm.off(ni, nj, nk) : 0;
         if constexpr (STAG) {
-            ue[0] = a.gmu.data[off];
+            ue[0] = node_velocity(a.gmu, a.gnm, off, moff, 0);
         } else {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) ue[d] = a.gmu.data[off + d * a.gmu.cs];
+            for (int d = 0; d < 3; ++d) ue[d] = node_velocity(a.gmu, a.gnm, off, moff, d);
         }
     }, cp);
     const int64_t cb = a.c.base(p);
@@ -155,14 +157,15 @@ extern "C" int pic_apic_p2g(pic_context* ctx, int order, int entity, int dtype,
     return launch_status();
 }
 
-extern "C" int pic_apic_g2p(pic_context* ctx, int order, int entity, int dtype,
-                            const pic_mesh* mesh, int64_t np, const pic_pfield* x,
-                            const pic_grid* u_i, const pic_pfield* c_p,
-                            const pic_bins* bins)
+static int apic_g2p_impl(pic_context* ctx, int order, int entity, int dtype,
+                         const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                         const pic_grid* u_i, const pic_grid* m_i, const pic_pfield* c_p,
+                         const pic_bins* bins)
 {
     if (!ctx || !mesh || !(mesh->h > 0.0) || order < 1 || order > 3 ||
         !valid_entity(entity) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
-        np > int64_t(UINT32_MAX) || !valid_grid(u_i))
+        np > int64_t(UINT32_MAX) || !valid_grid(u_i) || (m_i && !valid_grid(m_i)) ||
+        (m_i && !same_window(*m_i, *u_i)))
         return PIC_ERR_ARG;
     if (np > 0 && (!valid_pfield(x) || !valid_pfield(c_p))) return PIC_ERR_ARG;
     if (bins && !bins->perm) return PIC_ERR_ARG;
@@ -171,7 +174,7 @@ extern "C" int pic_apic_g2p(pic_context* ctx, int order, int entity, int dtype,
     if (!g.ok) return PIC_ERR_HIP;
     const bool stag = stag_dim(entity) >= 0;
     if (bins) {
-        int rc = apic_g2p_brick(ctx, order, entity, dtype, *mesh, np, *x, *u_i, *c_p, *bins);
+        int rc = apic_g2p_brick(ctx, order, entity, dtype, *mesh, np, *x, *u_i, m_i, *c_p, *bins);
         if (rc != PIC_ERR_UNSUPPORTED) return rc;
     }
     TimedLaunch tl(ctx, "apic_g2p");
@@ -180,12 +183,31 @@ extern "C" int pic_apic_g2p(pic_context* ctx, int order, int entity, int dtype,
         G2PGatherLaunch<double> L{
             make_args<double>(ctx, entity, *mesh, np, x, nullptr, c_p, nullptr, *u_i, perm),
             ctx->stream};
+        if (m_i) L.a.gnm = make_gview<double>(*m_i);
         dispatch_order_stag<double>(order, stag, L);
     } else {
         G2PGatherLaunch<float> L{
             make_args<float>(ctx, entity, *mesh, np, x, nullptr, c_p, nullptr, *u_i, perm),
             ctx->stream};
+        if (m_i) L.a.gnm = make_gview<float>(*m_i);
         dispatch_order_stag<float>(order, stag, L);
     }
     return launch_status();
+}
+
+extern "C" int pic_apic_g2p(pic_context* ctx, int order, int entity, int dtype,
+                            const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                            const pic_grid* u_i, const pic_pfield* c_p,
+                            const pic_bins* bins)
+{
+    return apic_g2p_impl(ctx, order, entity, dtype, mesh, np, x, u_i, nullptr, c_p, bins);
+}
+
+extern "C" int pic_apic_g2p_normalized(pic_context* ctx, int order, int entity, int dtype,
+                                       const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                                       const pic_grid* m_i, const pic_grid* mu_i,
+                                       const pic_pfield* c_p, const pic_bins* bins)
+{
+    if (!m_i) return PIC_ERR_ARG;
+    return apic_g2p_impl(ctx, order, entity, dtype, mesh, np, x, mu_i, m_i, c_p, bins);
 }

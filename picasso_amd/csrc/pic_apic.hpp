@@ -15,6 +15,7 @@ struct ApicArgs {
     const uint32_t* perm;  // sorted slot -> particle, or null (identity)
     PField<T> x, m, c;     // positions (3), mass (1), c_p (12)
     GView<T> gm, gmu;      // grid mass, grid momentum (or velocity for g2p)
+    GView<T> gnm;          // g2p with fused normalisation: grid mass (else null data)
     int sdim;              // staggered component (entity dim)
     int* err;
 };
@@ -181,6 +182,7 @@ inline ApicArgs<T> make_args(pic_context* ctx, int entity, const pic_mesh& mesh,
     a.c = make_pfield<T>(*c);
     a.gm = gm ? make_gview<T>(*gm) : GView<T>{};
     a.gmu = make_gview<T>(gmu);
+    a.gnm = GView<T>{};
     a.sdim = stag_dim(entity) < 0 ? 0 : stag_dim(entity);
     a.err = ctx->d_err;
     return a;
@@ -212,7 +214,19 @@ int apic_p2g_brick(pic_context* ctx, int order, int entity, int dtype, const pic
                    int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
                    const pic_grid& gm, const pic_grid& gmu, const pic_bins& bins);
 int apic_g2p_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
-                   int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_pfield& c,
-                   const pic_bins& bins);
+                   int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
+                   const pic_pfield& c, const pic_bins& bins);
+
+// Grid velocity at a node: the stored value, or momentum / mass (0 where the
+// mass is 0) with the division in FP64 when normalisation is fused in.
+template <typename T>
+__device__ __forceinline__ T node_velocity(const GView<T>& gu, const GView<T>& gm, int64_t off,
+                                           int64_t moff, int d)
+{
+    const T q = gu.data[off + d * gu.cs];
+    if (!gm.data) return q;
+    const T mm = gm.data[moff];
+    return mm > T(0) ? T(double(q) / double(mm)) : T(0);
+}
 
 }  // namespace pic

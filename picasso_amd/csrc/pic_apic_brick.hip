@@ -1,7 +1,8 @@
 // APIC brick scheme for particles binned by pic_bin_sort.
 //
-// One workgroup per brick of BB^3 bins.  The bins are aligned with the
-// spline's knot base (node-centred dual cells for quadratic splines on
+// A workgroup processes bricks of BB^3 bins (looping over the list of
+// non-empty bricks when the sort provided one).  The bins are aligned with
+// the spline's knot base (node-centred dual cells for quadratic splines on
 // nodes, mesh cells for linear/cubic), so every particle of a bin shares one
 // stencil and a brick's particles touch a fixed (BB+order)^3 node footprint.
 //
@@ -14,8 +15,9 @@
 //   The footprint is flushed to HBM with one global atomic per touched node
 //   and component (nodes shared with neighbouring bricks receive one partial
 //   per brick).
-// g2p: the footprint of grid values is staged in LDS with row loads; each
-//   work-item then reconstructs one particle from LDS.
+// g2p: the footprint of grid values is staged in LDS with row loads (from
+//   the velocity, or from momentum / mass when normalisation is fused in);
+//   each work-item then reconstructs one particle from LDS.
 // A particle whose stencil is not where its bin says (positions changed
 // since the sort) takes a slow path straight to global memory, so results
 // never depend on the bins being current.
@@ -27,12 +29,15 @@ namespace pic {
 constexpr int BB = 8;               // bins per brick edge
 constexpr int BVOL = BB * BB * BB;  // bins per brick
 constexpr int BT = 256;             // work-items per brick workgroup
+constexpr int MAX_BRICK_WG = 2048;  // workgroups looping over the active list
 
 template <typename T>
 struct BrickArgs {
     ApicArgs<T> a;
     const uint32_t* keys;
     const uint32_t* bin_start;
+    const uint32_t* active;  // [count, ids...] of non-empty bricks, or null
+    uint32_t nbricks;
     int32_t cell_lo[3];
     int32_t nb[3];
     int32_t knot_off;  // first knot index = bin index + knot_off
@@ -47,6 +52,18 @@ __device__ __forceinline__ void brick_origin(const BrickArgs<T>& g, uint32_t b, 
     f0[0] = g.cell_lo[0] + bi * BB + g.knot_off;
     f0[1] = g.cell_lo[1] + bj * BB + g.knot_off;
     f0[2] = g.cell_lo[2] + bk * BB + g.knot_off;
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t brick_count(const BrickArgs<T>& g)
+{
+    return g.active ? g.active[0] : g.nbricks;
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t brick_id(const BrickArgs<T>& g, uint32_t w)
+{
+    return g.active ? g.active[1 + w] : w;
 }
 
 // LDS accumulation; FP32 keeps the exact rounding error of every add.
@@ -78,18 +95,15 @@ __device__ void p2g_global(const ApicArgs<T>& a, const ApicParticle<T>& pt,
 }
 
 template <int ORDER, typename T, bool REGACC>
-__global__ __launch_bounds__(BT) void k_apic_p2g_brick(BrickArgs<T> g)
+__device__ __forceinline__ void p2g_brick(const BrickArgs<T>& g, uint32_t b, T* accv, T* accc,
+                                          uint32_t* cst)
 {
     constexpr int NK = ORDER + 1;
     constexpr int NN = NK * NK * NK;
     constexpr int FN = BB + ORDER;
     constexpr int FV = FN * FN * FN;
     constexpr bool COMP = std::is_same<T, float>::value;
-    __shared__ T accv[4 * FV];
-    __shared__ T accc[COMP ? 4 * FV : 1];
-    __shared__ uint32_t cst[BVOL + 1];
 
-    const uint32_t b = blockIdx.x;
     const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
     if (beg == end) return;
     int f0[3];
@@ -200,14 +214,27 @@ __global__ __launch_bounds__(BT) void k_apic_p2g_brick(BrickArgs<T> g)
     }
 }
 
-template <int ORDER, typename T>
-__global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
+template <int ORDER, typename T, bool REGACC>
+__global__ __launch_bounds__(BT) void k_apic_p2g_brick(BrickArgs<T> g)
 {
     constexpr int FN = BB + ORDER;
     constexpr int FV = FN * FN * FN;
-    __shared__ T us[3 * FV];
+    constexpr bool COMP = std::is_same<T, float>::value;
+    __shared__ T accv[4 * FV];
+    __shared__ T accc[COMP ? 4 * FV : 1];
+    __shared__ uint32_t cst[BVOL + 1];
+    const uint32_t nw = brick_count(g);
+    for (uint32_t w = blockIdx.x; w < nw; w += gridDim.x) {
+        p2g_brick<ORDER, T, REGACC>(g, brick_id(g, w), accv, accc, cst);
+        __syncthreads();
+    }
+}
 
-    const uint32_t b = blockIdx.x;
+template <int ORDER, typename T>
+__device__ __forceinline__ void g2p_brick(const BrickArgs<T>& g, uint32_t b, T* us)
+{
+    constexpr int FN = BB + ORDER;
+    constexpr int FV = FN * FN * FN;
     const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
     if (beg == end) return;
     int f0[3];
@@ -217,8 +244,10 @@ __global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
         const int ni = f0[0] + q / (FN * FN), nj = f0[1] + (q / FN) % FN, nk = f0[2] + q % FN;
         const bool in = gu.covers(0, ni, ni) && gu.covers(1, nj, nj) && gu.covers(2, nk, nk);
         const int64_t off = in ? gu.off(ni, nj, nk) : 0;
+        const int64_t moff = in && g.a.gnm.data ? g.a.gnm.off(ni, nj, nk) : 0;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) us[d * FV + q] = in ? gu.data[off + d * gu.cs] : T(0);
+        for (int d = 0; d < 3; ++d)
+            us[d * FV + q] = in ? node_velocity(gu, g.a.gnm, off, moff, d) : T(0);
     }
     __syncthreads();
 
@@ -244,14 +273,29 @@ __global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
                 continue;
             }
             apic_g2p_particle<ORDER, false>(g.a, sd, [&](int i, int j, int k, T* ue) {
-                const int64_t off = gu.off(sd.s0[0] + i, sd.s0[1] + j, sd.s0[2] + k);
+                const int ni = sd.s0[0] + i, nj = sd.s0[1] + j, nk = sd.s0[2] + k;
+                const int64_t off = gu.off(ni, nj, nk);
+                const int64_t moff = g.a.gnm.data ? g.a.gnm.off(ni, nj, nk) : 0;
 #pragma unroll
-                for (int d = 0; d < 3; ++d) ue[d] = gu.data[off + d * gu.cs];
+                for (int d = 0; d < 3; ++d) ue[d] = node_velocity(gu, g.a.gnm, off, moff, d);
             }, cp);
         }
         const int64_t cb = g.a.c.base(p);
 #pragma unroll
         for (int r = 0; r < 12; ++r) g.a.c.at(cb, r) = cp[r];
+    }
+}
+
+template <int ORDER, typename T>
+__global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
+{
+    constexpr int FN = BB + ORDER;
+    constexpr int FV = FN * FN * FN;
+    __shared__ T us[3 * FV];
+    const uint32_t nw = brick_count(g);
+    for (uint32_t w = blockIdx.x; w < nw; w += gridDim.x) {
+        g2p_brick<ORDER, T>(g, brick_id(g, w), us);
+        __syncthreads();
     }
 }
 
@@ -271,21 +315,26 @@ template <typename T>
 static bool make_brick_args(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
                             int64_t np, const pic_pfield& x, const pic_pfield* m,
                             const pic_pfield& c, const pic_grid* gm, const pic_grid& gmu,
-                            const pic_bins& bins, BrickArgs<T>& g, int64_t& nb)
+                            const pic_grid* gmass, const pic_bins& bins, BrickArgs<T>& g,
+                            unsigned& nwg)
 {
     for (int d = 0; d < 3; ++d)
         if (bins.geom.brick[d] != BB) return false;
     if (!bins.keys || !bins.bin_start) return false;
     if (!aligned_knot_offset(order, entity, bins.geom.dual, &g.knot_off)) return false;
-    nb = pic_bin_count(&bins.geom);
+    const int64_t nb = pic_bin_count(&bins.geom);
     if (nb <= 0) return false;
     g.a = make_args<T>(ctx, entity, mesh, np, &x, m, &c, gm, gmu, bins.perm);
+    if (gmass) g.a.gnm = make_gview<T>(*gmass);
     g.keys = bins.keys;
     g.bin_start = bins.bin_start;
+    g.active = bins.active;
+    g.nbricks = uint32_t(nb);
     for (int d = 0; d < 3; ++d) {
         g.cell_lo[d] = bins.geom.cell_lo[d];
         g.nb[d] = (bins.geom.ncell[d] + BB - 1) / BB;
     }
+    nwg = bins.active ? unsigned(std::min<int64_t>(nb, MAX_BRICK_WG)) : unsigned(nb);
     return true;
 }
 
@@ -293,52 +342,54 @@ int apic_p2g_brick(pic_context* ctx, int order, int entity, int dtype, const pic
                    int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
                    const pic_grid& gm, const pic_grid& gmu, const pic_bins& bins)
 {
-    int64_t nb = 0;
+    unsigned nwg = 0;
     hipStream_t st = ctx->stream;
     if (dtype == PIC_F32) {
         BrickArgs<float> g;
-        if (!make_brick_args<float>(ctx, order, entity, mesh, np, x, &m, c, &gm, gmu, bins, g, nb))
+        if (!make_brick_args<float>(ctx, order, entity, mesh, np, x, &m, c, &gm, gmu, nullptr,
+                                    bins, g, nwg))
             return PIC_ERR_UNSUPPORTED;
         TimedLaunch tl(ctx, "apic_p2g");
-        if (order == 1) k_apic_p2g_brick<1, float, true><<<(unsigned)nb, BT, 0, st>>>(g);
-        else if (order == 2) k_apic_p2g_brick<2, float, true><<<(unsigned)nb, BT, 0, st>>>(g);
-        else k_apic_p2g_brick<3, float, false><<<(unsigned)nb, BT, 0, st>>>(g);
+        if (order == 1) k_apic_p2g_brick<1, float, true><<<nwg, BT, 0, st>>>(g);
+        else if (order == 2) k_apic_p2g_brick<2, float, true><<<nwg, BT, 0, st>>>(g);
+        else k_apic_p2g_brick<3, float, false><<<nwg, BT, 0, st>>>(g);
     } else {
         BrickArgs<double> g;
-        if (!make_brick_args<double>(ctx, order, entity, mesh, np, x, &m, c, &gm, gmu, bins, g, nb))
+        if (!make_brick_args<double>(ctx, order, entity, mesh, np, x, &m, c, &gm, gmu, nullptr,
+                                     bins, g, nwg))
             return PIC_ERR_UNSUPPORTED;
         TimedLaunch tl(ctx, "apic_p2g");
-        if (order == 1) k_apic_p2g_brick<1, double, false><<<(unsigned)nb, BT, 0, st>>>(g);
-        else if (order == 2) k_apic_p2g_brick<2, double, false><<<(unsigned)nb, BT, 0, st>>>(g);
-        else k_apic_p2g_brick<3, double, false><<<(unsigned)nb, BT, 0, st>>>(g);
+        if (order == 1) k_apic_p2g_brick<1, double, false><<<nwg, BT, 0, st>>>(g);
+        else if (order == 2) k_apic_p2g_brick<2, double, false><<<nwg, BT, 0, st>>>(g);
+        else k_apic_p2g_brick<3, double, false><<<nwg, BT, 0, st>>>(g);
     }
     return launch_status();
 }
 
 int apic_g2p_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
-                   int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_pfield& c,
-                   const pic_bins& bins)
+                   int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
+                   const pic_pfield& c, const pic_bins& bins)
 {
-    int64_t nb = 0;
+    unsigned nwg = 0;
     hipStream_t st = ctx->stream;
     if (dtype == PIC_F32) {
         BrickArgs<float> g;
-        if (!make_brick_args<float>(ctx, order, entity, mesh, np, x, nullptr, c, nullptr, gu, bins,
-                                    g, nb))
+        if (!make_brick_args<float>(ctx, order, entity, mesh, np, x, nullptr, c, nullptr, gu,
+                                    gmass, bins, g, nwg))
             return PIC_ERR_UNSUPPORTED;
         TimedLaunch tl(ctx, "apic_g2p");
-        if (order == 1) k_apic_g2p_brick<1, float><<<(unsigned)nb, BT, 0, st>>>(g);
-        else if (order == 2) k_apic_g2p_brick<2, float><<<(unsigned)nb, BT, 0, st>>>(g);
-        else k_apic_g2p_brick<3, float><<<(unsigned)nb, BT, 0, st>>>(g);
+        if (order == 1) k_apic_g2p_brick<1, float><<<nwg, BT, 0, st>>>(g);
+        else if (order == 2) k_apic_g2p_brick<2, float><<<nwg, BT, 0, st>>>(g);
+        else k_apic_g2p_brick<3, float><<<nwg, BT, 0, st>>>(g);
     } else {
         BrickArgs<double> g;
-        if (!make_brick_args<double>(ctx, order, entity, mesh, np, x, nullptr, c, nullptr, gu, bins,
-                                     g, nb))
+        if (!make_brick_args<double>(ctx, order, entity, mesh, np, x, nullptr, c, nullptr, gu,
+                                     gmass, bins, g, nwg))
             return PIC_ERR_UNSUPPORTED;
         TimedLaunch tl(ctx, "apic_g2p");
-        if (order == 1) k_apic_g2p_brick<1, double><<<(unsigned)nb, BT, 0, st>>>(g);
-        else if (order == 2) k_apic_g2p_brick<2, double><<<(unsigned)nb, BT, 0, st>>>(g);
-        else k_apic_g2p_brick<3, double><<<(unsigned)nb, BT, 0, st>>>(g);
+        if (order == 1) k_apic_g2p_brick<1, double><<<nwg, BT, 0, st>>>(g);
+        else if (order == 2) k_apic_g2p_brick<2, double><<<nwg, BT, 0, st>>>(g);
+        else k_apic_g2p_brick<3, double><<<nwg, BT, 0, st>>>(g);
     }
     return launch_status();
 }

@@ -52,35 +52,75 @@ TimedLaunch::~TimedLaunch()
     if (idx != SIZE_MAX) (void)hipEventRecord(ctx->recs[idx].stop, ctx->stream);
 }
 
-// Grid-window fill with zeros.  A kernel rather than hipMemsetAsync: windows
+// Grid-window fills with zeros.  Kernels rather than hipMemsetAsync: windows
 // are strided, and memset nodes misbehave in captured graphs.
+
+// Strided window: x runs along k (one wave per 64 nodes), y along j, z over
+// (i, component).
 template <typename T>
-__global__ void k_zero_grid(GView<T> g, int ncomp)
+__global__ __launch_bounds__(256) void k_zero_grid3(GView<T> g)
 {
-    const int64_t n12 = int64_t(g.n[1]) * g.n[2];
-    const int64_t total = int64_t(g.n[0]) * n12 * ncomp;
-    for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < total;
-         t += int64_t(gridDim.x) * blockDim.x) {
-        // k fastest so consecutive lanes write consecutive k for unit stride.
-        const int64_t k = t % g.n[2];
-        const int64_t r = t / g.n[2];
-        const int64_t j = r % g.n[1];
-        const int64_t r2 = r / g.n[1];
-        const int64_t i = r2 % g.n[0];
-        const int64_t c = r2 / g.n[0];
-        g.data[i * g.s[0] + j * g.s[1] + k * g.s[2] + c * g.cs] = T(0);
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int i = blockIdx.z % g.n[0];
+    const int c = blockIdx.z / g.n[0];
+    if (k < g.n[2] && j < g.n[1])
+        g.data[int64_t(i) * g.s[0] + int64_t(j) * g.s[1] + int64_t(k) * g.s[2] +
+               int64_t(c) * g.cs] = T(0);
+}
+
+// Window whose elements form one dense range: 16-byte vector stores.
+__global__ __launch_bounds__(256) void k_zero_flat(uint4* p, int64_t n16, char* tail, int ntail)
+{
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n16;
+         q += int64_t(gridDim.x) * blockDim.x)
+        p[q] = z;
+    if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
+}
+
+// Do the window's elements (ncomp components) cover one dense range?
+static bool dense_window(const pic_grid& g, int ncomp, int64_t* count)
+{
+    int64_t ext[4] = {g.n[0], g.n[1], g.n[2], ncomp};
+    int64_t str[4] = {g.stride[0], g.stride[1], g.stride[2], ncomp > 1 ? g.cstride : 0};
+    int order[4] = {0, 1, 2, 3};
+    for (int a = 0; a < 4; ++a)
+        for (int b = a + 1; b < 4; ++b)
+            if (str[order[b]] < str[order[a]]) std::swap(order[a], order[b]);
+    int64_t expect = 1, total = 1;
+    for (int q = 0; q < 4; ++q) {
+        const int d = order[q];
+        if (ext[d] == 1) continue;
+        if (str[d] != expect) return false;
+        expect *= ext[d];
+        total *= ext[d];
     }
+    *count = total;
+    return true;
 }
 
 int zero_grid(pic_context* ctx, int dtype, const pic_grid& g, int ncomp)
 {
-    const int64_t total = int64_t(g.n[0]) * g.n[1] * g.n[2] * ncomp;
-    unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 65536);
     TimedLaunch tl(ctx, "zero_grid");
+    const size_t esz = dtype == PIC_F64 ? 8 : 4;
+    int64_t count = 0;
+    if (dense_window(g, ncomp, &count) && (reinterpret_cast<uintptr_t>(g.data) & 15) == 0) {
+        const int64_t bytes = count * int64_t(esz);
+        const int64_t n16 = bytes / 16;
+        const unsigned blocks = (unsigned)std::max<int64_t>(
+            1, std::min<int64_t>((n16 + 255) / 256, 8192));
+        k_zero_flat<<<blocks, 256, 0, ctx->stream>>>(static_cast<uint4*>(g.data), n16,
+                                                      static_cast<char*>(g.data) + n16 * 16,
+                                                      int(bytes - n16 * 16));
+        return launch_status();
+    }
+    const dim3 block(64, 4);
+    const dim3 grid((g.n[2] + 63) / 64, (g.n[1] + 3) / 4, unsigned(g.n[0]) * unsigned(ncomp));
     if (dtype == PIC_F64)
-        k_zero_grid<double><<<blocks, 256, 0, ctx->stream>>>(make_gview<double>(g), ncomp);
+        k_zero_grid3<double><<<grid, block, 0, ctx->stream>>>(make_gview<double>(g));
     else
-        k_zero_grid<float><<<blocks, 256, 0, ctx->stream>>>(make_gview<float>(g), ncomp);
+        k_zero_grid3<float><<<grid, block, 0, ctx->stream>>>(make_gview<float>(g));
     return launch_status();
 }
 

@@ -214,6 +214,22 @@ __device__ void brick_segment_sort(uint32_t b, uint32_t beg, uint32_t n, uint32_
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) perm[beg + i] = items[i];
 }
 
+// List of non-empty bricks: flags -> exclusive scan -> ids in increasing order.
+__global__ __launch_bounds__(256) void k_nonempty_flags(const uint32_t* count, int64_t nb,
+                                                        uint32_t* flags)
+{
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b <= nb) flags[b] = (b < nb && count[b] > 0u) ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_compact_active(const uint32_t* flags, const uint32_t* pos,
+                                                        int64_t nb, uint32_t* active)
+{
+    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (b < nb && flags[b]) active[1 + pos[b]] = uint32_t(b);
+    if (b == nb) active[0] = pos[nb];
+}
+
 __global__ __launch_bounds__(256) void k_brick_sort(const uint32_t* start, const uint32_t* tmp,
                                                     const uint32_t* keys_in, uint32_t bvol,
                                                     uint32_t* big, uint32_t* perm,
@@ -271,7 +287,7 @@ KeyGeom make_key_geom(const pic_mesh& mesh, const pic_binning& b)
 
 extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, int64_t np,
                             const pic_pfield* x, const pic_binning* geom, uint32_t* keys,
-                            uint32_t* perm, uint32_t* bin_start)
+                            uint32_t* perm, uint32_t* bin_start, uint32_t* active)
 {
     if (!ctx || !mesh || !(mesh->h > 0.0) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
         np >= int64_t(UINT32_MAX) || !geom || !bin_start)
@@ -283,10 +299,12 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
     DeviceGuard dg(ctx->device);
     if (!dg.ok) return PIC_ERR_HIP;
 
-    // scratch: count[nb+1] | cursor[nb] | scan tmp | keys_in[np] | tmp[np] | big[np]
+    // scratch: count[nb+1] | cursor[nb] | scan tmp | keys_in[np] | tmp[w] | big[w]
+    // with w = max(np, nb+1) (tmp and big also hold the active-brick flags).
     const int64_t scan_words = scan_tmp_words(nb + 1);
     const int64_t head = (nb + 1) + nb + scan_words;
-    const size_t bytes = size_t(head + 3 * np) * 4;
+    const int64_t w = std::max<int64_t>(np, nb + 1);
+    const size_t bytes = size_t(head + np + 2 * w) * 4;
     uint32_t* sc = static_cast<uint32_t*>(scratch(ctx, bytes));
     if (!sc) return PIC_ERR_ALLOC;
     uint32_t* count = sc;
@@ -294,7 +312,7 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
     uint32_t* scan_tmp = cursor + nb;
     uint32_t* keys_in = sc + head;
     uint32_t* tmp = keys_in + np;
-    uint32_t* big = tmp + np;
+    uint32_t* big = tmp + w;
     hipStream_t st = ctx->stream;
 
     TimedLaunch tl(ctx, "bin_sort");
@@ -310,6 +328,15 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
                                                    ctx->d_err);
     }
     exclusive_scan(st, count, bin_start, nb + 1, scan_tmp);
+    if (active) {
+        // flags and their scan live in tmp and big before the scatter uses them.
+        uint32_t* flags = tmp;
+        uint32_t* pos = big;
+        const unsigned fb2 = blocks_for(nb + 1, 256);
+        k_nonempty_flags<<<fb2, 256, 0, st>>>(count, nb, flags);
+        exclusive_scan(st, flags, pos, nb + 1, scan_tmp);
+        k_compact_active<<<fb2, 256, 0, st>>>(flags, pos, nb, active);
+    }
     if (np > 0) {
         k_bin_scatter<<<blocks_for(np, 256), 256, 0, st>>>(np, keys_in, g.bvol, bin_start, cursor,
                                                             tmp);

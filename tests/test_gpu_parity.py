@@ -388,3 +388,46 @@ def test_empty_inputs(pa):
     res = pa.bin_sort(gu.ctx(), mesh, x, pa.Binning((0, 0, 0), (4, 4, 4), (2, 2, 2), 0))
     gu.ctx().synchronize()
     assert np.all(gu.host(res.bin_start) == 0)
+
+
+@pytest.mark.parametrize("binned", [False, True])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_g2p_normalized_matches_normalize_then_g2p(pa, wl0, binned, dtype):
+    """Fused normalisation + g2p against the oracle's normalise then g2p."""
+    import gpu_util as gu
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    f32 = dtype == "f32"
+    xh = wl0.x.astype(np.float32).astype(np.float64) if f32 else wl0.x
+    mh = wl0.m.astype(np.float32).astype(np.float64) if f32 else wl0.m
+    ch = wl0.c.astype(np.float32).astype(np.float64) if f32 else wl0.c
+    lo, n = wl0.node_window
+    mesh = pa.Mesh(wl0.low, wl0.h)
+    x = gu.soa(xh, tdt)
+    bins = pa.bin_sort(gu.ctx(), mesh, x, _binning(pa, wl0)) if binned else None
+    sd = pa.create_spline(0, 2, mesh, x, bins)
+    gm, gmu = _node_grids(pa, lo, n, tdt)
+    c = gu.soa(ch.reshape(-1, 12), tdt)
+    pa.APIC.p2g(gu.ctx(), gu.soa(mh[:, None], tdt), c, gm, gmu, sd, zero_first=True)
+    pa.APIC.g2p_normalized(gu.ctx(), gm, gmu, c, sd)
+    gu.ctx().synchronize()
+    rm, rmu, ru, rc = _oracle_step(wl0, xh, mh, ch, 2)
+    tol = F32_TOL if f32 else F64_TOL
+    cc = gu.host(c).reshape(-1, 4, 3)
+    assert gu.rel_inf(cc[:, 0], rc[:, 0]) < tol
+    assert gu.rel_inf(cc[:, 1:], rc[:, 1:]) < tol * 10
+
+
+def test_active_brick_list(pa, wl0):
+    """The sort's list of non-empty bricks equals the bricks with particles."""
+    import gpu_util as gu
+    from picasso_amd import synth
+    wl = synth.apic_sphere(32, 4, radius=0.3)
+    x = gu.soa(wl.x)
+    geom = _binning(pa, wl)
+    res = pa.bin_sort(gu.ctx(), pa.Mesh(wl.low, wl.h), x, geom)
+    gu.ctx().synchronize()
+    start = gu.host(res.bin_start).astype(np.int64)
+    want = np.nonzero(np.diff(start) > 0)[0]
+    act = gu.host(res.active)
+    assert act[0] == len(want)
+    np.testing.assert_array_equal(act[1:1 + len(want)], want)
