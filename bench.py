@@ -29,7 +29,7 @@ sys.path.insert(0, ROOT)
 METRIC = ("APIC p2g+g2p particles/s @256³×8ppc, FP32-comp vs FP64 at stated err; "
           "HBM GB/s")
 HBM_PEAK_GBS = 8000.0
-BRICK = (8, 8, 8)
+BRICK = (4, 4, 4)
 
 
 def log(msg):

@@ -12,72 +12,85 @@
 namespace pic {
 
 // ---------------------------------------------------------------------------
-// p2g, scatter scheme.
+// p2g, scatter scheme.  Grid-stride over particles, or over a device-side
+// list of sorted slots (list[0] = count) left by the brick kernels.
 template <int ORDER, bool STAG, typename T>
 __global__ __launch_bounds__(256) void k_apic_p2g_scatter(ApicArgs<T> a)
 {
-    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= a.np) return;
-    const int64_t p = a.perm ? int64_t(a.perm[s]) : s;
-
-    ApicParticle<T> pt;
-    load_apic_particle<STAG>(a, p, pt);
-    Spline<ORDER, T> sd;
-    eval_spline<ORDER, T, ORDER == 1, ORDER != 1>(a.lat, pt.x, sd);
-    if (!stencil_inside(sd, a.gmu)) {
-        atomicOr(a.err, ERRBIT_BOUNDS);
-        return;
-    }
-    apic_p2g_particle<ORDER, STAG>(a, pt, sd, [&](int i, int j, int k, const T* mu, T m) {
-        i += sd.s0[0];
-        j += sd.s0[1];
-        k += sd.s0[2];
-        const int64_t off = a.gmu.off(i, j, k);
-        if constexpr (STAG) {
-            atomic_add(a.gmu.data + off, mu[0]);
-        } else {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) atomic_add(a.gmu.data + off + d * a.gmu.cs, mu[d]);
+    const int64_t n = a.list ? int64_t(a.list[0]) : a.np;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = a.list ? int64_t(a.list[1 + i]) : i;
+        const int64_t p = a.perm ? int64_t(a.perm[s]) : s;
+        ApicParticle<T> pt;
+        load_apic_particle<STAG>(a, p, pt);
+        Spline<ORDER, T> sd;
+        eval_spline<ORDER, T, ORDER == 1, ORDER != 1>(a.lat, pt.x, sd);
+        if (!stencil_inside(sd, a.gmu)) {
+            atomicOr(a.err, ERRBIT_BOUNDS);
+            continue;
         }
-        atomic_add(a.gm.data + a.gm.off(i, j, k), m);
-    });
+        apic_p2g_particle<ORDER, STAG>(a, pt, sd, [&](int i, int j, int k, const T* mu, T m) {
+            i += sd.s0[0];
+            j += sd.s0[1];
+            k += sd.s0[2];
+            const int64_t off = a.gmu.off(i, j, k);
+            if constexpr (STAG) {
+                atomic_add(a.gmu.data + off, mu[0]);
+            } else {
+#pragma unroll
+                for (int d = 0; d < 3; ++d) atomic_add(a.gmu.data + off + d * a.gmu.cs, mu[d]);
+            }
+            atomic_add(a.gm.data + a.gm.off(i, j, k), m);
+        });
+    }
 }
 
 // ---------------------------------------------------------------------------
-// g2p, gather (any particle order).
+// g2p, gather (any particle order), same iteration scheme.
 template <int ORDER, bool STAG, typename T>
 __global__ __launch_bounds__(256) void k_apic_g2p_gather(ApicArgs<T> a)
 {
-    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= a.np) return;
-    const int64_t p = a.perm ? int64_t(a.perm[s]) : s;
-    const int64_t pb = a.x.base(p);
-    T xp[3] = {a.x.at(pb, 0), a.x.at(pb, 1), a.x.at(pb, 2)};
-    Spline<ORDER, T> sd;
-    eval_spline<ORDER, T, false, true>(a.lat, xp, sd);
-    if (!stencil_inside(sd, a.gmu)) {
-        atomicOr(a.err, ERRBIT_BOUNDS);
-        return;
-    }
-    T cp[12];
-    apic_g2p_particle<ORDER, STAG>(a, sd, [&](int i, int j, int k, T* ue) {
-        const int ni = sd.s0[0] + i, nj = sd.s0[1] + j, nk = sd.s0[2] + k;
-        const int64_t off = a.gmu.off(ni, nj, nk);
-        const int64_t moff = a.gnm.data ? a.gnm.off(ni, nj, nk) : 0;
-        if constexpr (STAG) {
-            ue[0] = node_velocity(a.gmu, a.gnm, off, moff, 0);
-        } else {
-#pragma unroll
-            for (int d = 0; d < 3; ++d) ue[d] = node_velocity(a.gmu, a.gnm, off, moff, d);
+    const int64_t n = a.list ? int64_t(a.list[0]) : a.np;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = a.list ? int64_t(a.list[1 + i]) : i;
+        const int64_t p = a.perm ? int64_t(a.perm[s]) : s;
+        const int64_t pb = a.x.base(p);
+        T xp[3] = {a.x.at(pb, 0), a.x.at(pb, 1), a.x.at(pb, 2)};
+        Spline<ORDER, T> sd;
+        eval_spline<ORDER, T, false, true>(a.lat, xp, sd);
+        if (!stencil_inside(sd, a.gmu)) {
+            atomicOr(a.err, ERRBIT_BOUNDS);
+            continue;
         }
-    }, cp);
-    const int64_t cb = a.c.base(p);
+        T cp[12];
+        apic_g2p_particle<ORDER, STAG>(a, sd, [&](int i, int j, int k, T* ue) {
+            const int ni = sd.s0[0] + i, nj = sd.s0[1] + j, nk = sd.s0[2] + k;
+            const int64_t off = a.gmu.off(ni, nj, nk);
+            const int64_t moff = a.gnm.data ? a.gnm.off(ni, nj, nk) : 0;
+            if constexpr (STAG) {
+                ue[0] = node_velocity(a.gmu, a.gnm, off, moff, 0);
+            } else {
 #pragma unroll
-    for (int r = 0; r < 12; ++r) a.c.at(cb, r) = cp[r];
+                for (int d = 0; d < 3; ++d) ue[d] = node_velocity(a.gmu, a.gnm, off, moff, d);
+            }
+        }, cp);
+        const int64_t cb = a.c.base(p);
+#pragma unroll
+        for (int r = 0; r < 12; ++r) a.c.at(cb, r) = cp[r];
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Host dispatch.
+// Host dispatch.  With a slot list the element count lives on the device,
+// so a fixed grid strides over it.
+constexpr unsigned LIST_BLOCKS = 512;
+
+static unsigned launch_blocks(const void* list, int64_t np)
+{
+    return list ? LIST_BLOCKS : blocks_for(np, 256);
+}
 
 template <typename T>
 struct P2GScatterLaunch {
@@ -86,7 +99,7 @@ struct P2GScatterLaunch {
     template <int O, bool S, typename U>
     void operator()()
     {
-        k_apic_p2g_scatter<O, S, U><<<blocks_for(a.np, 256), 256, 0, st>>>(a);
+        k_apic_p2g_scatter<O, S, U><<<launch_blocks(a.list, a.np), 256, 0, st>>>(a);
     }
 };
 
@@ -97,9 +110,56 @@ struct G2PGatherLaunch {
     template <int O, bool S, typename U>
     void operator()()
     {
-        k_apic_g2p_gather<O, S, U><<<blocks_for(a.np, 256), 256, 0, st>>>(a);
+        k_apic_g2p_gather<O, S, U><<<launch_blocks(a.list, a.np), 256, 0, st>>>(a);
     }
 };
+
+int apic_p2g_list(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
+                  int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
+                  const pic_grid& gm, const pic_grid& gmu, const uint32_t* perm,
+                  const uint32_t* list)
+{
+    const bool stag = stag_dim(entity) >= 0;
+    TimedLaunch tl(ctx, "apic_p2g_stale");
+    if (dtype == PIC_F64) {
+        P2GScatterLaunch<double> L{make_args<double>(ctx, entity, mesh, np, &x, &m, &c, &gm, gmu,
+                                                     perm),
+                                   ctx->stream};
+        L.a.list = list;
+        dispatch_order_stag<double>(order, stag, L);
+    } else {
+        P2GScatterLaunch<float> L{make_args<float>(ctx, entity, mesh, np, &x, &m, &c, &gm, gmu,
+                                                   perm),
+                                  ctx->stream};
+        L.a.list = list;
+        dispatch_order_stag<float>(order, stag, L);
+    }
+    return launch_status();
+}
+
+int apic_g2p_list(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
+                  int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
+                  const pic_pfield& c, const uint32_t* perm, const uint32_t* list)
+{
+    const bool stag = stag_dim(entity) >= 0;
+    TimedLaunch tl(ctx, "apic_g2p_stale");
+    if (dtype == PIC_F64) {
+        G2PGatherLaunch<double> L{
+            make_args<double>(ctx, entity, mesh, np, &x, nullptr, &c, nullptr, gu, perm),
+            ctx->stream};
+        L.a.list = list;
+        if (gmass) L.a.gnm = make_gview<double>(*gmass);
+        dispatch_order_stag<double>(order, stag, L);
+    } else {
+        G2PGatherLaunch<float> L{
+            make_args<float>(ctx, entity, mesh, np, &x, nullptr, &c, nullptr, gu, perm),
+            ctx->stream};
+        L.a.list = list;
+        if (gmass) L.a.gnm = make_gview<float>(*gmass);
+        dispatch_order_stag<float>(order, stag, L);
+    }
+    return launch_status();
+}
 
 }  // namespace pic
 

@@ -13,6 +13,7 @@ struct ApicArgs {
     Lattice lat;
     int64_t np;
     const uint32_t* perm;  // sorted slot -> particle, or null (identity)
+    const uint32_t* list;  // [count, slots...] to process instead of 0..np-1, or null
     PField<T> x, m, c;     // positions (3), mass (1), c_p (12)
     GView<T> gm, gmu;      // grid mass, grid momentum (or velocity for g2p)
     GView<T> gnm;          // g2p with fused normalisation: grid mass (else null data)
@@ -74,22 +75,30 @@ __device__ __forceinline__ void apic_p2g_particle(const ApicArgs<T>& a,
         for (int d = 0; d < NC; ++d)
 #pragma unroll
             for (int q = 0; q < 3; ++q) DB[d][q] = Dinv * pt.B[d][q];
+        // v_n = u_p + D^{-1} B_p d_n, built up one dimension at a time so
+        // each node costs one multiply-add per component.
 #pragma unroll
-        for (int i = 0; i < NK; ++i)
+        for (int i = 0; i < NK; ++i) {
+            T vi[NC];
+#pragma unroll
+            for (int d = 0; d < NC; ++d) vi[d] = pt.c0[d] + DB[d][0] * sd.d[0][i];
+            const T wim = sd.w[0][i] * pt.m;
 #pragma unroll
             for (int j = 0; j < NK; ++j) {
-                const T wij = sd.w[0][i] * sd.w[1][j];
+                T vij[NC];
+#pragma unroll
+                for (int d = 0; d < NC; ++d) vij[d] = vi[d] + DB[d][1] * sd.d[1][j];
+                const T wijm = wim * sd.w[1][j];
 #pragma unroll
                 for (int k = 0; k < NK; ++k) {
-                    const T wm = wij * sd.w[2][k] * pt.m;
+                    const T wm = wijm * sd.w[2][k];
                     T mu[NC];
 #pragma unroll
-                    for (int d = 0; d < NC; ++d)
-                        mu[d] = wm * (pt.c0[d] + (DB[d][0] * sd.d[0][i] + DB[d][1] * sd.d[1][j] +
-                                                  DB[d][2] * sd.d[2][k]));
+                    for (int d = 0; d < NC; ++d) mu[d] = wm * (vij[d] + DB[d][2] * sd.d[2][k]);
                     emit(i, j, k, mu, wm);
                 }
             }
+        }
     } else {
         // Gradient form, no inertial scaling (Picasso_APIC.hpp:287-318).
 #pragma unroll
@@ -177,6 +186,7 @@ inline ApicArgs<T> make_args(pic_context* ctx, int entity, const pic_mesh& mesh,
     a.lat = make_lattice(mesh, entity);
     a.np = np;
     a.perm = perm;
+    a.list = nullptr;
     a.x = make_pfield<T>(*x);
     a.m = m ? make_pfield<T>(*m) : PField<T>{};
     a.c = make_pfield<T>(*c);
@@ -207,6 +217,19 @@ inline void dispatch_order_stag(int order, bool stag, Launch& L)
         break;
     }
 }
+
+// Scatter/gather over a device-side slot list (pic_apic.hip).
+int apic_p2g_list(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
+                  int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
+                  const pic_grid& gm, const pic_grid& gmu, const uint32_t* perm,
+                  const uint32_t* list);
+int apic_g2p_list(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
+                  int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
+                  const pic_pfield& c, const uint32_t* perm, const uint32_t* list);
+
+#ifndef PIC_F64_P2G_REGACC
+#define PIC_F64_P2G_REGACC true
+#endif
 
 // Brick scheme (pic_apic_brick.hip).  Return PIC_ERR_UNSUPPORTED to fall
 // back to the scatter/gather kernels.

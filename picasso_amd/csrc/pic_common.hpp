@@ -278,6 +278,35 @@ __device__ __forceinline__ bool stencil_inside(const Spline<ORDER, T>& sd,
            g.covers(2, sd.s0[2], sd.s0[2] + NK - 1);
 }
 
+// Sum over the four lanes of a quad (lanes 4q..4q+3), result in all four.
+// DPP quad permutations keep this on the VALU (a generic shuffle would go
+// through the LDS crossbar).
+template <int CTRL>
+__device__ __forceinline__ int dpp_i32(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+
+__device__ __forceinline__ float quad_sum(float v)
+{
+    v += __int_as_float(dpp_i32<0xB1>(__float_as_int(v)));  // quad_perm [1,0,3,2]
+    v += __int_as_float(dpp_i32<0x4E>(__float_as_int(v)));  // quad_perm [2,3,0,1]
+    return v;
+}
+
+__device__ __forceinline__ double quad_sum(double v)
+{
+    auto perm = [](double x, auto ctrl) {
+        const long long b = __double_as_longlong(x);
+        const int lo = dpp_i32<decltype(ctrl)::value>(int(b & 0xffffffffll));
+        const int hi = dpp_i32<decltype(ctrl)::value>(int(b >> 32));
+        return __longlong_as_double((long long)(unsigned)lo | ((long long)hi << 32));
+    };
+    v += perm(v, std::integral_constant<int, 0xB1>{});
+    v += perm(v, std::integral_constant<int, 0x4E>{});
+    return v;
+}
+
 // Hardware floating-point atomic add (global_atomic_add_f32/_f64 on gfx950).
 __device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
 __device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }

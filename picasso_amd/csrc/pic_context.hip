@@ -100,6 +100,14 @@ static bool dense_window(const pic_grid& g, int ncomp, int64_t* count)
     return true;
 }
 
+__global__ void k_store_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+int set_u32(pic_context* ctx, uint32_t* p, uint32_t v)
+{
+    k_store_u32<<<1, 1, 0, ctx->stream>>>(p, v);
+    return launch_status();
+}
+
 int zero_grid(pic_context* ctx, int dtype, const pic_grid& g, int ncomp)
 {
     TimedLaunch tl(ctx, "zero_grid");

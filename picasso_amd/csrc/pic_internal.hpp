@@ -87,6 +87,10 @@ inline unsigned blocks_for(int64_t n, int threads)
     return (unsigned)((n + threads - 1) / threads);
 }
 
+// Store one value into device memory on the stream (a kernel, so it can be
+// captured in graphs).
+int set_u32(pic_context* ctx, uint32_t* p, uint32_t v);
+
 // Zero a grid window on the stream (used by PIC_P2G_ZERO_FIRST).
 int zero_grid(pic_context* ctx, int dtype, const pic_grid& g, int ncomp);
 

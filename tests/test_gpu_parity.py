@@ -187,7 +187,8 @@ def test_bin_sort_out_of_range_reported(pa, wl0):
 @pytest.mark.parametrize("order", [1, 2, 3])
 @pytest.mark.parametrize("entity", [0, 1])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-def test_brick_transfers_vs_oracle(pa, wl0, order, entity, dtype):
+@pytest.mark.parametrize("edge", [4, 8])
+def test_brick_transfers_vs_oracle(pa, wl0, order, entity, dtype, edge):
     """Binned p2g/g2p (brick kernels) against the oracle."""
     import gpu_util as gu
     tdt = torch.float64 if dtype == "f64" else torch.float32
@@ -200,7 +201,7 @@ def test_brick_transfers_vs_oracle(pa, wl0, order, entity, dtype):
     x = gu.soa(xh, tdt)
     m = gu.soa(mh[:, None], tdt)
     c = gu.soa(ch.reshape(-1, 12), tdt)
-    bins = pa.bin_sort(gu.ctx(), mesh, x, _binning(pa, wl0, order, entity))
+    bins = pa.bin_sort(gu.ctx(), mesh, x, _binning(pa, wl0, order, entity, (edge,) * 3))
     sd = pa.create_spline(entity, order, mesh, x, bins)
     gm, gmu = _node_grids(pa, lo, n, tdt)
     pa.APIC.p2g(gu.ctx(), m, c, gm, gmu, sd, zero_first=True)
