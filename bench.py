@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-shuffled", action="store_true",
+                    help="skip the reference step on shuffled storage (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the transfer path is built and timed on a single "
@@ -169,16 +171,18 @@ def main():
     families = ["bin_sort", "zero_grid", "apic_p2g", "max_mass", "normalize", "apic_g2p"]
 
     s32 = Step(pa, ctx, wl, torch.float32, x32, m32, c32)
-    # One step on the shuffled storage order, for reference.
-    s32.run()
-    s32.restore()
-    torch.cuda.synchronize()
-    t_sh = time.perf_counter()
-    s32.run()
-    torch.cuda.synchronize()
-    shuffled_ms = (time.perf_counter() - t_sh) * 1e3
-    s32.restore()
-    log(f"fp32 step on shuffled storage: {shuffled_ms:.3f} ms")
+    shuffled_ms = None
+    if not args.skip_shuffled:
+        # One step on the shuffled storage order, for reference.
+        s32.run()
+        s32.restore()
+        torch.cuda.synchronize()
+        t_sh = time.perf_counter()
+        s32.run()
+        torch.cuda.synchronize()
+        shuffled_ms = (time.perf_counter() - t_sh) * 1e3
+        s32.restore()
+        log(f"fp32 step on shuffled storage: {shuffled_ms:.3f} ms")
     s32.reorder()
     ms32, kt32 = time_steps(s32, args.steps, args.warmup, ctx, families)
     log(f"fp32: {ms32:.3f} ms/step, kernels {json.dumps(kt32)}")

@@ -227,10 +227,6 @@ int apic_g2p_list(pic_context* ctx, int order, int entity, int dtype, const pic_
                   int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
                   const pic_pfield& c, const uint32_t* perm, const uint32_t* list);
 
-#ifndef PIC_F64_P2G_REGACC
-#define PIC_F64_P2G_REGACC true
-#endif
-
 // Brick scheme (pic_apic_brick.hip).  Return PIC_ERR_UNSUPPORTED to fall
 // back to the scatter/gather kernels.
 int apic_p2g_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,

@@ -1,22 +1,22 @@
 // APIC brick scheme for particles binned by pic_bin_sort.
 //
-// One workgroup per non-empty brick of BB^3 bins (BB = 4 or 8).  The bins
-// are aligned with the spline's knot base (node-centred dual cells for
-// quadratic splines on nodes, mesh cells for linear/cubic), so every
+// The bins are aligned with the spline's knot base (node-centred dual cells
+// for quadratic splines on nodes, mesh cells for linear/cubic), so every
 // particle of a bin shares one stencil and a brick's particles touch a fixed
-// (BB+order)^3 node footprint.
+// (edge + order)^3 node footprint.  Workgroups walk the non-empty bricks in
+// list order, taking the next one from a counter (neighbouring bricks run at
+// the same time and share their halo nodes in L2; uneven bricks balance).
 //
-// p2g: the brick's particles are staged through LDS in chunks with
-//   coalesced loads (all loads of a chunk in flight at once).  KL
-//   consecutive work-items then share a bin: each takes every KL-th staged
-//   particle of it, sums the contributions in registers, the group reduces
-//   its sums across lanes, and one lane adds the bin sums into the LDS node
-//   footprint.  FP32 LDS sums are compensated: the returned old value of the
-//   LDS atomic gives the exact rounding error of each add (two-sum), which is
-//   accumulated in a second LDS array and folded in at the flush.  The
-//   footprint is flushed to HBM with one global atomic per touched node and
-//   component (nodes shared with neighbouring bricks receive one partial per
-//   brick).
+// p2g (4^3-bin bricks): four work-items per bin, one per grid component
+//   (mass, momentum x/y/z).  Each walks every particle of its bin, evaluates
+//   the spline and accumulates its component at the bin's NK^3 stencil nodes
+//   in registers, without any cross-lane or LDS traffic.  The bin sums are
+//   then stored to LDS (plain stores, one slot per bin and stencil offset),
+//   and each work-item of a node of the footprint gathers the up-to-NK^3
+//   sums that land on it, adding them in FP64 (exact enough to stand in for
+//   compensated FP32 summation, and an FP64 add costs one FP32 issue slot on
+//   CDNA4), then adds the node total into HBM with one atomic per component.
+//   Nodes shared with neighbouring bricks receive one partial per brick.
 // g2p: the footprint of grid values is staged in LDS with row loads (from
 //   the velocity, or from momentum / mass when normalisation is fused in);
 //   each work-item then reconstructs one particle from LDS.
@@ -28,9 +28,8 @@
 
 namespace pic {
 
-constexpr int BT = 256;    // work-items per brick workgroup
-constexpr int CHUNK = 512;  // particles staged in LDS at a time
-constexpr int NPF = 16;     // staged scalars per particle: x(3), m, c_p(12)
+constexpr int BT = 256;  // work-items per brick workgroup
+constexpr int PB = 4;    // p2g brick edge in bins: 64 bins x 4 components = BT
 
 template <typename T>
 struct BrickArgs {
@@ -38,6 +37,7 @@ struct BrickArgs {
     const uint32_t* keys;
     const uint32_t* bin_start;
     const uint32_t* active;  // [count, ids...] of non-empty bricks, or null
+    uint32_t* walk;          // brick-walk counter, zero at launch
     uint32_t* stale;         // [count, slots...] of particles for the slow path
     uint32_t nbricks;
     int32_t cell_lo[3];
@@ -56,17 +56,40 @@ __device__ __forceinline__ void brick_origin(const BrickArgs<T>& g, uint32_t b, 
     f0[2] = g.cell_lo[2] + bk * BB + g.knot_off;
 }
 
-// Brick handled by this workgroup, or false when it has nothing to do.
+// Walk indices run over the active list when there is one, else over every
+// brick (empty ones are skipped).
 template <typename T>
-__device__ __forceinline__ bool my_brick(const BrickArgs<T>& g, uint32_t* b)
+__device__ __forceinline__ uint32_t walk_count(const BrickArgs<T>& g)
 {
-    if (g.active) {
-        if (blockIdx.x >= g.active[0]) return false;
-        *b = g.active[1 + blockIdx.x];
-    } else {
-        *b = blockIdx.x;
+    return g.active ? g.active[0] : g.nbricks;
+}
+
+template <typename T>
+__device__ __forceinline__ uint32_t walk_brick(const BrickArgs<T>& g, uint32_t t)
+{
+    return g.active ? g.active[1 + t] : t;
+}
+
+// Drive body(brick) over the walk.  A workgroup's first index is its
+// blockIdx; each later one is requested from the counter when the previous
+// brick starts, so the request's latency hides behind that brick.  Every
+// workgroup ends after drawing an index past the end.  `slot` is a two-word
+// LDS buffer (alternating, so no extra barrier is needed when a brick is
+// skipped).
+template <typename T, typename Body>
+__device__ __forceinline__ void walk_bricks(const BrickArgs<T>& g, uint32_t* slot, Body&& body)
+{
+    const uint32_t nw = walk_count(g);
+    uint32_t t = blockIdx.x;
+    for (int it = 0; t < nw; ++it) {
+        uint32_t nxt = 0;
+        if (threadIdx.x == 0) nxt = gridDim.x + atomicAdd(g.walk, 1u);
+        const uint32_t b = walk_brick(g, t);
+        if (g.bin_start[b] != g.bin_start[b + 1]) body(b);
+        if (threadIdx.x == 0) slot[it & 1] = nxt;
+        __syncthreads();
+        t = slot[it & 1];
     }
-    return g.bin_start[*b] != g.bin_start[*b + 1];
 }
 
 template <typename T>
@@ -76,266 +99,198 @@ __device__ __forceinline__ void push_stale(const BrickArgs<T>& g, uint32_t s)
     g.stale[1 + at] = s;
 }
 
-// Exact rounding error of the FP32 sum old + x (two-sum; contraction cannot
-// apply: there are no products).
-__device__ __forceinline__ float two_sum_err(float old, float x)
-{
-    const float s = old + x;
-    const float bb = s - old;
-    return (old - (s - bb)) + (x - bb);
-}
+// LDS layout of the p2g bin sums: per component round, [component][stencil
+// offset][bin], each component padded by 16 words so the four components a
+// bin group stores at once start in different banks.  Components are staged
+// in 1, 2 or 4 rounds to keep the buffer near 34 KB (five workgroups per
+// compute unit).
+template <int ORDER, typename T>
+struct P2GShape {
+    static constexpr int NK = ORDER + 1;
+    static constexpr int NN = NK * NK * NK;
+    static constexpr int FN = PB + ORDER;
+    static constexpr int FV = FN * FN * FN;
+    static constexpr int BINS = PB * PB * PB;
+    static constexpr int CSTRIDE = NN * BINS + 16;
+    static constexpr int BUDGET = 34 * 1024;
+    static constexpr int NROUND = 4 * CSTRIDE * int(sizeof(T)) <= BUDGET   ? 1
+                                  : 2 * CSTRIDE * int(sizeof(T)) <= BUDGET ? 2
+                                                                            : 4;
+    static constexpr int CPR = 4 / NROUND;  // components per round
+};
 
-// Compensated LDS accumulation: value += x, comp += rounding error of that add.
-__device__ __forceinline__ void lds_two_sum(float* v, float* c, float x)
+// Accumulate one particle's component into the bin's stencil sums.  v and
+// B0..B2 are row `comp - 1` of u_p and B_p for a momentum component; the
+// mass component passes v = 1 and B = 0, which yields exactly the weighted
+// mass with the same instruction stream.
+template <int ORDER, typename T>
+__device__ __forceinline__ void p2g_component(const Spline<ORDER, T>& sd, T m, T v, T B0, T B1,
+                                              T B2, T Dinv, T* acc)
 {
-    const float old = atomicAdd(v, x);
-    atomicAdd(c, two_sum_err(old, x));
-}
-
-// Exclusive scan of cst[1..n] in place (cst[0] = 0 stays), by one wave.
-__device__ __forceinline__ void scan_counts(uint32_t* cst, int n)
-{
-    if (threadIdx.x < 64) {
-        const int lane = threadIdx.x;
-        uint32_t carry = 0;
-        for (int c0 = 1; c0 <= n; c0 += 64) {
-            const uint32_t v = (c0 + lane <= n) ? cst[c0 + lane] : 0u;
-            uint32_t inc = v;
+    constexpr int NK = ORDER + 1;
+    if constexpr (ORDER >= 2) {
+        // m w_i w_j w_k (v + D^-1 (B0 d0_i + B1 d1_j + B2 d2_k))
+        // (Picasso_APIC.hpp:110-138), factored per dimension.
+        const T DB0 = Dinv * B0, DB1 = Dinv * B1, DB2 = Dinv * B2;
+        T ai[NK], wim[NK], bj[NK], wck[NK];
 #pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(inc, o);
-                if (lane >= o) inc += y;
-            }
-            if (c0 + lane <= n) cst[c0 + lane] = carry + inc;
-            carry += __shfl(inc, 63);
+        for (int i = 0; i < NK; ++i) {
+            ai[i] = v + DB0 * sd.d[0][i];
+            wim[i] = sd.w[0][i] * m;
+            bj[i] = DB1 * sd.d[1][i];
+            wck[i] = sd.w[2][i] * (DB2 * sd.d[2][i]);
         }
+#pragma unroll
+        for (int i = 0; i < NK; ++i)
+#pragma unroll
+            for (int j = 0; j < NK; ++j) {
+                const T P = wim[i] * sd.w[1][j];
+                const T Tij = ai[i] + bj[j];
+#pragma unroll
+                for (int k = 0; k < NK; ++k)
+                    acc[(i * NK + j) * NK + k] += P * (sd.w[2][k] * Tij + wck[k]);
+            }
+    } else {
+        // Gradient form m (w v + B grad w) (Picasso_APIC.hpp:287-318),
+        // factored per dimension.
+        T al[NK], be[NK];
+#pragma unroll
+        for (int i = 0; i < NK; ++i) {
+            be[i] = m * sd.w[0][i];
+            al[i] = m * (v * sd.w[0][i] + B0 * sd.g[0][i]);
+        }
+#pragma unroll
+        for (int i = 0; i < NK; ++i)
+#pragma unroll
+            for (int j = 0; j < NK; ++j) {
+                const T gam = al[i] * sd.w[1][j] + be[i] * (B1 * sd.g[1][j]);
+                const T del = be[i] * sd.w[1][j] * B2;
+#pragma unroll
+                for (int k = 0; k < NK; ++k)
+                    acc[(i * NK + j) * NK + k] += sd.w[2][k] * gam + sd.g[2][k] * del;
+            }
     }
 }
 
-template <int BB, int ORDER, typename T, bool REGACC>
-__global__ __launch_bounds__(BT) void k_apic_p2g_brick(BrickArgs<T> g)
+template <int ORDER, typename T>
+__device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b, T* sums,
+                                              uint32_t* rng)
 {
-    constexpr int BVOL = BB * BB * BB;
-    constexpr int NK = ORDER + 1;
-    constexpr int NN = NK * NK * NK;
-    constexpr int FN = BB + ORDER;
-    constexpr int FV = FN * FN * FN;
-    constexpr bool COMP = std::is_same<T, float>::value;
-    constexpr int KL = REGACC ? 4 : 1;
-    // Small bricks give every wave a private footprint copy: bin sums are
-    // then added with plain LDS read-modify-writes (LDS float atomics run at
-    // a small fraction of the LDS rate), and the copies are merged at the
-    // flush.
-    constexpr bool PRIV = REGACC && BB == 4;
-    constexpr int NCOPY = PRIV ? BT / 64 : 1;
-    __shared__ T accv[NCOPY * 4 * FV];
-    __shared__ T accc[COMP ? NCOPY * 4 * FV : 1];
-    __shared__ T pbuf[NPF * CHUNK];
-    __shared__ uint32_t cst[BVOL + 1];
-
-    uint32_t b;
-    if (!my_brick(g, &b)) return;
+    using S = P2GShape<ORDER, T>;
+    constexpr int NK = S::NK, NN = S::NN, FN = S::FN, FV = S::FV, BINS = S::BINS;
+    const ApicArgs<T>& a = g.a;
     const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
     int f0[3];
-    brick_origin<BB>(g, b, f0);
+    brick_origin<PB>(g, b, f0);
 
-    for (int q = threadIdx.x; q < NCOPY * 4 * FV; q += BT) {
-        accv[q] = T(0);
-        if constexpr (COMP) accc[q] = T(0);
+    // Bin ranges [rng[l], rng[BINS + l]) from the sorted keys (empty: 0, 0).
+    for (int l = threadIdx.x; l < 2 * BINS; l += BT) rng[l] = 0u;
+    __syncthreads();
+    const uint32_t kb = b * BINS;
+    for (uint32_t s = beg + threadIdx.x; s < end; s += BT) {
+        const uint32_t key = g.keys[s];
+        if (s == beg || g.keys[s - 1] != key) rng[key - kb] = s;
+        if (s + 1 == end || g.keys[s + 1] != key) rng[BINS + key - kb] = s + 1;
     }
-    for (int q = threadIdx.x; q <= BVOL; q += BT) cst[q] = 0u;
-    __syncthreads();
-    // Bin ranges inside the brick (relative to beg) from the sorted keys.
-    const uint32_t kb = b * BVOL;
-    for (uint32_t s = beg + threadIdx.x; s < end; s += BT) atomicAdd(&cst[g.keys[s] - kb + 1], 1u);
-    __syncthreads();
-    scan_counts(cst, BVOL);
     __syncthreads();
 
-    const ApicArgs<T>& a = g.a;
-    for (uint32_t c0 = beg; c0 < end; c0 += CHUNK) {
-        const uint32_t cn = min(uint32_t(CHUNK), end - c0);
-        // Stage the chunk: consecutive work-items load consecutive particles.
-        for (uint32_t i = threadIdx.x; i < cn; i += BT) {
-            ApicParticle<T> pt;
-            load_apic_particle<false>(a, int64_t(a.perm[c0 + i]), pt);
-            pbuf[0 * CHUNK + i] = pt.x[0];
-            pbuf[1 * CHUNK + i] = pt.x[1];
-            pbuf[2 * CHUNK + i] = pt.x[2];
-            pbuf[3 * CHUNK + i] = pt.m;
+    const int l = int(threadIdx.x) >> 2, comp = int(threadIdx.x) & 3;
+    const int li = l / (PB * PB), lj = (l / PB) % PB, lk = l % PB;
+    const int e0 = f0[0] + li, e1 = f0[1] + lj, e2 = f0[2] + lk;
+    const int d = comp > 0 ? comp - 1 : 0;
+    const T hh = (T)a.lat.h;
+    const T Dinv = T(ORDER == 3 ? 3.0 : 4.0) / (hh * hh);  // APIC::inertialScaling (:33-47)
+    T acc[NN];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                pbuf[(4 + d) * CHUNK + i] = pt.c0[d];
-#pragma unroll
-                for (int q = 0; q < 3; ++q) pbuf[(7 + d * 3 + q) * CHUNK + i] = pt.B[d][q];
-            }
+    for (int n = 0; n < NN; ++n) acc[n] = T(0);
+    const uint32_t s1 = rng[BINS + l];
+    for (uint32_t s = rng[l]; s < s1; ++s) {
+        const int64_t p = int64_t(a.perm[s]);
+        const int64_t xb = a.x.base(p);
+        const T xp[3] = {a.x.at(xb, 0), a.x.at(xb, 1), a.x.at(xb, 2)};
+        const T m = a.m.at(a.m.base(p), 0);
+        // B(d, q) = c_p(1 + q, d) (Picasso_APIC.hpp:94-101).
+        const int64_t cb = a.c.base(p);
+        T v = a.c.at(cb, d), B0 = a.c.at(cb, 3 + d), B1 = a.c.at(cb, 6 + d),
+          B2 = a.c.at(cb, 9 + d);
+        if (comp == 0) {
+            v = T(1);
+            B0 = B1 = B2 = T(0);
         }
-        __syncthreads();
-        // Bins overlapping the chunk, KL work-items per bin.
-        const uint32_t lo = c0 - beg, hi = lo + cn;
-        const int lb = int(g.keys[c0] - kb), le = int(g.keys[c0 + cn - 1] - kb);
-        for (int l0 = lb; l0 <= le; l0 += BT / KL) {
-            const int l = l0 + int(threadIdx.x) / KL;
-            const int ql = int(threadIdx.x) % KL;
-            const bool mine = l <= le;
-            const uint32_t s0 = mine ? max(cst[l], lo) : 0u;
-            const uint32_t s1 = mine ? min(cst[l + 1], hi) : 0u;
-            const int li = l / (BB * BB), lj = (l / BB) % BB, lk = l % BB;
-            const int e0 = f0[0] + li, e1 = f0[1] + lj, e2 = f0[2] + lk;
-            const int lbase = (li * FN + lj) * FN + lk;
-            T acc[REGACC ? NN : 1][4];
-            if constexpr (REGACC) {
-#pragma unroll
-                for (int n = 0; n < NN; ++n) acc[n][0] = acc[n][1] = acc[n][2] = acc[n][3] = T(0);
-            }
-            for (uint32_t s = s0 + ql; s < s1; s += KL) {
-                const uint32_t i = s - lo;
-                ApicParticle<T> pt;
-                pt.x[0] = pbuf[0 * CHUNK + i];
-                pt.x[1] = pbuf[1 * CHUNK + i];
-                pt.x[2] = pbuf[2 * CHUNK + i];
-                pt.m = pbuf[3 * CHUNK + i];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    pt.c0[d] = pbuf[(4 + d) * CHUNK + i];
-#pragma unroll
-                    for (int q = 0; q < 3; ++q) pt.B[d][q] = pbuf[(7 + d * 3 + q) * CHUNK + i];
-                }
-                Spline<ORDER, T> sd;
-                eval_spline<ORDER, T, ORDER == 1, ORDER != 1>(a.lat, pt.x, sd);
-                if (sd.s0[0] != e0 || sd.s0[1] != e1 || sd.s0[2] != e2) {
-                    push_stale(g, beg + s);
-                    continue;
-                }
-                apic_p2g_particle<ORDER, false>(a, pt, sd, [&](int i, int j, int k, const T* mu, T m) {
-                    if constexpr (REGACC) {
-                        T* a4 = acc[(i * NK + j) * NK + k];
-                        a4[0] += m;
-                        a4[1] += mu[0];
-                        a4[2] += mu[1];
-                        a4[3] += mu[2];
-                    } else {
-                        const int q = lbase + (i * FN + j) * FN + k;
-                        if constexpr (COMP) {
-                            lds_two_sum(&accv[q], &accc[q], m);
-#pragma unroll
-                            for (int d = 0; d < 3; ++d)
-                                lds_two_sum(&accv[(d + 1) * FV + q], &accc[(d + 1) * FV + q], mu[d]);
-                        } else {
-                            atomicAdd(&accv[q], m);
-#pragma unroll
-                            for (int d = 0; d < 3; ++d) atomicAdd(&accv[(d + 1) * FV + q], mu[d]);
-                        }
-                    }
-                });
-            }
-            if constexpr (REGACC) {
-#pragma unroll
-                for (int n = 0; n < NN; ++n)
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        static_assert(KL == 4, "bin groups are DPP quads");
-                        acc[n][c] = quad_sum(acc[n][c]);
-                    }
-                if constexpr (PRIV) {
-                    // Work-item ql of the group adds component ql into its
-                    // wave's copy, one stencil node per step.  Within a step
-                    // the wave's bins hit distinct nodes, and the steps stay
-                    // in program order (the compiler may not move LDS
-                    // accesses across the empty asm), so plain
-                    // read-modify-writes are race-free and the two-sum sees
-                    // the exact running value.
-                    const int wv = int(threadIdx.x) / 64;
-                    T* vp = accv + (wv * 4 + ql) * FV + lbase;
-                    T* cp = accc + (COMP ? (wv * 4 + ql) * FV + lbase : 0);
-                    if (s1 > s0) {
-#pragma unroll
-                        for (int n = 0; n < NN; ++n) {
-                            const int i = n / (NK * NK), j = (n / NK) % NK, k = n % NK;
-                            const int q = (i * FN + j) * FN + k;
-                            const T x = ql == 0 ? acc[n][0]
-                                                : (ql == 1 ? acc[n][1] : (ql == 2 ? acc[n][2] : acc[n][3]));
-                            asm volatile("" ::: "memory");
-                            const T old = vp[q];
-                            vp[q] = old + x;
-                            if constexpr (COMP) cp[q] += two_sum_err(old, x);
-                        }
-                    }
-                    asm volatile("" ::: "memory");
-                } else if (ql == 0 && s1 > s0) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        if constexpr (COMP) {
-                            // All returning adds of a component are issued
-                            // before any result is used, so their latencies
-                            // overlap.
-                            T old[NN];
-#pragma unroll
-                            for (int n = 0; n < NN; ++n) {
-                                const int i = n / (NK * NK), j = (n / NK) % NK, k = n % NK;
-                                old[n] = atomicAdd(&accv[c * FV + lbase + (i * FN + j) * FN + k],
-                                                   acc[n][c]);
-                            }
-#pragma unroll
-                            for (int n = 0; n < NN; ++n) {
-                                const int i = n / (NK * NK), j = (n / NK) % NK, k = n % NK;
-                                atomicAdd(&accc[c * FV + lbase + (i * FN + j) * FN + k],
-                                          two_sum_err(old[n], acc[n][c]));
-                            }
-                        } else {
-#pragma unroll
-                            for (int n = 0; n < NN; ++n) {
-                                const int i = n / (NK * NK), j = (n / NK) % NK, k = n % NK;
-                                atomicAdd(&accv[c * FV + lbase + (i * FN + j) * FN + k], acc[n][c]);
-                            }
-                        }
-                    }
-                }
-            }
-        }
-        __syncthreads();
-    }
-
-    // Flush the footprint (merging the per-wave copies, compensated).
-    for (int q = threadIdx.x; q < FV; q += BT) {
-        const int fi = q / (FN * FN), fj = (q / FN) % FN, fk = q % FN;
-        T v[4];
-        bool any = false;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            T s = accv[c * FV + q];
-            T e = COMP ? accc[c * FV + q] : T(0);
-#pragma unroll
-            for (int w = 1; w < NCOPY; ++w) {
-                const T x = accv[(w * 4 + c) * FV + q];
-                if constexpr (COMP) e += two_sum_err(s, x) + accc[(w * 4 + c) * FV + q];
-                s += x;
-            }
-            v[c] = s + e;
-            any = any || v[c] != T(0);
-        }
-        if (!any) continue;
-        const int ni = f0[0] + fi, nj = f0[1] + fj, nk = f0[2] + fk;
-        if (!(a.gmu.covers(0, ni, ni) && a.gmu.covers(1, nj, nj) && a.gmu.covers(2, nk, nk))) {
-            atomicOr(a.err, ERRBIT_BOUNDS);
+        Spline<ORDER, T> sd;
+        eval_spline<ORDER, T, ORDER == 1, ORDER != 1>(a.lat, xp, sd);
+        if (sd.s0[0] != e0 || sd.s0[1] != e1 || sd.s0[2] != e2) {
+            if (comp == 0) push_stale(g, s);
             continue;
         }
-        atomic_add(a.gm.data + a.gm.off(ni, nj, nk), v[0]);
-        const int64_t off = a.gmu.off(ni, nj, nk);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) atomic_add(a.gmu.data + off + d * a.gmu.cs, v[d + 1]);
+        p2g_component<ORDER, T>(sd, m, v, B0, B1, B2, Dinv, acc);
     }
+
+    for (int r = 0; r < S::NROUND; ++r) {
+        const int c_lo = r * S::CPR;
+        if (comp >= c_lo && comp < c_lo + S::CPR) {
+            T* dst = sums + (comp - c_lo) * S::CSTRIDE + l;
+#pragma unroll
+            for (int n = 0; n < NN; ++n) dst[n * BINS] = acc[n];
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < FV; q += BT) {
+            const int fi = q / (FN * FN), fj = (q / FN) % FN, fk = q % FN;
+            double tot[S::CPR];
+#pragma unroll
+            for (int cc = 0; cc < S::CPR; ++cc) tot[cc] = 0.0;
+#pragma unroll
+            for (int oi = 0; oi < NK; ++oi)
+#pragma unroll
+                for (int oj = 0; oj < NK; ++oj)
+#pragma unroll
+                    for (int ok = 0; ok < NK; ++ok) {
+                        const int bi = fi - oi, bj = fj - oj, bk = fk - ok;
+                        if (bi < 0 || bi >= PB || bj < 0 || bj >= PB || bk < 0 || bk >= PB)
+                            continue;
+                        const T* src = sums + ((oi * NK + oj) * NK + ok) * BINS +
+                                       (bi * PB + bj) * PB + bk;
+#pragma unroll
+                        for (int cc = 0; cc < S::CPR; ++cc) tot[cc] += double(src[cc * S::CSTRIDE]);
+                    }
+            bool any = false;
+#pragma unroll
+            for (int cc = 0; cc < S::CPR; ++cc) any = any || tot[cc] != 0.0;
+            if (!any) continue;
+            const int ni = f0[0] + fi, nj = f0[1] + fj, nk = f0[2] + fk;
+            if (!(a.gmu.covers(0, ni, ni) && a.gmu.covers(1, nj, nj) && a.gmu.covers(2, nk, nk))) {
+                atomicOr(a.err, ERRBIT_BOUNDS);
+                continue;
+            }
+            const int64_t off = a.gmu.off(ni, nj, nk);
+#pragma unroll
+            for (int cc = 0; cc < S::CPR; ++cc) {
+                const int c = c_lo + cc;
+                const T v = T(tot[cc]);
+                if (v == T(0)) continue;
+                if (c == 0) atomic_add(a.gm.data + a.gm.off(ni, nj, nk), v);
+                else atomic_add(a.gmu.data + off + (c - 1) * a.gmu.cs, v);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int ORDER, typename T>
+__global__ __launch_bounds__(BT) void k_apic_p2g_brick(BrickArgs<T> g)
+{
+    using S = P2GShape<ORDER, T>;
+    __shared__ T sums[S::CPR * S::CSTRIDE];
+    __shared__ uint32_t rng[2 * S::BINS];
+    __shared__ uint32_t slot[2];
+    walk_bricks(g, slot, [&](uint32_t b) { p2g_one_brick<ORDER, T>(g, b, sums, rng); });
 }
 
 template <int BB, int ORDER, typename T>
-__global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
+__device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, uint32_t b, T* us)
 {
     constexpr int FN = BB + ORDER;
     constexpr int FV = FN * FN * FN;
-    __shared__ T us[3 * FV];
-
-    uint32_t b;
-    if (!my_brick(g, &b)) return;
     const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
     int f0[3];
     brick_origin<BB>(g, b, f0);
@@ -375,6 +330,16 @@ __global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
 #pragma unroll
         for (int r = 0; r < 12; ++r) a.c.at(cb, r) = cp[r];
     }
+    __syncthreads();
+}
+
+template <int BB, int ORDER, typename T>
+__global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
+{
+    constexpr int FN = BB + ORDER;
+    __shared__ T us[3 * FN * FN * FN];
+    __shared__ uint32_t slot[2];
+    walk_bricks(g, slot, [&](uint32_t b) { g2p_one_brick<BB, ORDER, T>(g, b, us); });
 }
 
 // Knot offset of the bins for (order, entity), or false when the bin
@@ -405,7 +370,7 @@ static bool make_brick_args(pic_context* ctx, int order, int entity, const pic_m
                             const pic_grid* gmass, const pic_bins& bins, BrickArgs<T>& g)
 {
     const int bb = brick_edge(bins.geom);
-    if (!bb || !bins.keys || !bins.bin_start) return false;
+    if (!bb || !bins.keys || !bins.bin_start || !bins.perm) return false;
     if (!aligned_knot_offset(order, entity, bins.geom.dual, &g.knot_off)) return false;
     const int64_t nb = pic_bin_count(&bins.geom);
     if (nb <= 0) return false;
@@ -419,39 +384,48 @@ static bool make_brick_args(pic_context* ctx, int order, int entity, const pic_m
         g.cell_lo[d] = bins.geom.cell_lo[d];
         g.nb[d] = (bins.geom.ncell[d] + bb - 1) / bb;
     }
-    g.stale = static_cast<uint32_t*>(scratch(ctx, size_t(np + 1) * sizeof(uint32_t)));
-    return g.stale != nullptr;
+    // scratch: walk counter | stale count | stale slots[np]
+    uint32_t* w = static_cast<uint32_t*>(scratch(ctx, size_t(np + 2) * sizeof(uint32_t)));
+    if (!w) return false;
+    g.walk = w;
+    g.stale = w + 1;
+    return fill_u32(ctx, w, 2, 0u) == PIC_OK;
 }
 
 template <typename T>
 struct P2GBrickLaunch {
+    pic_context* ctx;
     const BrickArgs<T>& g;
-    hipStream_t st;
-    template <int BB, int O, bool R>
+    template <int O>
     void go()
     {
-        k_apic_p2g_brick<BB, O, T, R><<<g.nbricks, BT, 0, st>>>(g);
+        const unsigned nbk = persistent_blocks(ctx, k_apic_p2g_brick<O, T>, BT, 0, g.nbricks);
+        k_apic_p2g_brick<O, T><<<nbk, BT, 0, ctx->stream>>>(g);
     }
-    template <int BB>
     void by_order(int order)
     {
-        constexpr bool F32 = std::is_same<T, float>::value;
-        if (order == 1) go<BB, 1, true>();
-        else if (order == 2) go<BB, 2, F32 ? true : PIC_F64_P2G_REGACC>();
-        else go<BB, 3, false>();
+        if (order == 1) go<1>();
+        else if (order == 2) go<2>();
+        else go<3>();
     }
 };
 
 template <typename T>
 struct G2PBrickLaunch {
+    pic_context* ctx;
     const BrickArgs<T>& g;
-    hipStream_t st;
+    template <int BB, int O>
+    void go()
+    {
+        const unsigned nbk = persistent_blocks(ctx, k_apic_g2p_brick<BB, O, T>, BT, 0, g.nbricks);
+        k_apic_g2p_brick<BB, O, T><<<nbk, BT, 0, ctx->stream>>>(g);
+    }
     template <int BB>
     void by_order(int order)
     {
-        if (order == 1) k_apic_g2p_brick<BB, 1, T><<<g.nbricks, BT, 0, st>>>(g);
-        else if (order == 2) k_apic_g2p_brick<BB, 2, T><<<g.nbricks, BT, 0, st>>>(g);
-        else k_apic_g2p_brick<BB, 3, T><<<g.nbricks, BT, 0, st>>>(g);
+        if (order == 1) go<BB, 1>();
+        else if (order == 2) go<BB, 2>();
+        else go<BB, 3>();
     }
 };
 
@@ -461,15 +435,13 @@ static int p2g_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
                          const pic_grid& gm, const pic_grid& gmu, const pic_bins& bins,
                          int dtype)
 {
+    if (brick_edge(bins.geom) != PB) return PIC_ERR_UNSUPPORTED;
     BrickArgs<T> g;
     if (!make_brick_args<T>(ctx, order, entity, mesh, np, x, &m, c, &gm, gmu, nullptr, bins, g))
         return PIC_ERR_UNSUPPORTED;
-    if (set_u32(ctx, g.stale, 0u)) return PIC_ERR_HIP;
     {
         TimedLaunch tl(ctx, "apic_p2g");
-        P2GBrickLaunch<T> L{g, ctx->stream};
-        if (brick_edge(bins.geom) == 4) L.template by_order<4>(order);
-        else L.template by_order<8>(order);
+        P2GBrickLaunch<T>{ctx, g}.by_order(order);
         if (launch_status()) return PIC_ERR_HIP;
     }
     return apic_p2g_list(ctx, order, entity, dtype, mesh, np, x, m, c, gm, gmu, bins.perm, g.stale);
@@ -484,10 +456,9 @@ static int g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
     if (!make_brick_args<T>(ctx, order, entity, mesh, np, x, nullptr, c, nullptr, gu, gmass, bins,
                             g))
         return PIC_ERR_UNSUPPORTED;
-    if (set_u32(ctx, g.stale, 0u)) return PIC_ERR_HIP;
     {
         TimedLaunch tl(ctx, "apic_g2p");
-        G2PBrickLaunch<T> L{g, ctx->stream};
+        G2PBrickLaunch<T> L{ctx, g};
         if (brick_edge(bins.geom) == 4) L.template by_order<4>(order);
         else L.template by_order<8>(order);
         if (launch_status()) return PIC_ERR_HIP;

@@ -100,11 +100,14 @@ static bool dense_window(const pic_grid& g, int ncomp, int64_t* count)
     return true;
 }
 
-__global__ void k_store_u32(uint32_t* p, uint32_t v) { *p = v; }
-
-int set_u32(pic_context* ctx, uint32_t* p, uint32_t v)
+__global__ void k_fill_few_u32(uint32_t* p, int n, uint32_t v)
 {
-    k_store_u32<<<1, 1, 0, ctx->stream>>>(p, v);
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+int fill_u32(pic_context* ctx, uint32_t* p, int n, uint32_t v)
+{
+    k_fill_few_u32<<<1, 64, 0, ctx->stream>>>(p, n, v);
     return launch_status();
 }
 
@@ -165,6 +168,11 @@ int pic_context_create(pic_context** out, int device)
     if (!g.ok) return PIC_ERR_HIP;
     pic_context* ctx = new pic_context();
     ctx->device = device;
+    if (hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, device) !=
+        hipSuccess) {
+        delete ctx;
+        return PIC_ERR_HIP;
+    }
     if (hipMalloc(&ctx->d_err, sizeof(int)) != hipSuccess) {
         delete ctx;
         return PIC_ERR_ALLOC;

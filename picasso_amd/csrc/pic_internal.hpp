@@ -6,13 +6,17 @@
 
 #include <stdint.h>
 
+#include <algorithm>
 #include <string>
+#include <map>
+#include <utility>
 #include <vector>
 
 #include "picasso_amd.h"
 
 struct pic_context {
     int device = 0;
+    int num_cu = 0;  // compute units of the device
     hipStream_t stream = nullptr;
     int* d_err = nullptr;  // device-side error word (ERRBIT_*)
 
@@ -28,6 +32,9 @@ struct pic_context {
     };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> event_pool;
+
+    // Resident workgroups per compute unit, per (kernel, dynamic LDS bytes).
+    std::map<std::pair<const void*, size_t>, int> occupancy;
 };
 
 namespace pic {
@@ -87,9 +94,31 @@ inline unsigned blocks_for(int64_t n, int threads)
     return (unsigned)((n + threads - 1) / threads);
 }
 
-// Store one value into device memory on the stream (a kernel, so it can be
+// Workgroup count for a persistent launch that walks `items` work items with
+// a grid-stride loop: as many workgroups as the device holds at once (from
+// the occupancy query, cached per kernel), and at most `items`.
+template <typename K>
+unsigned persistent_blocks(pic_context* ctx, K kernel, int threads, size_t dyn_lds, int64_t items)
+{
+    const auto key = std::make_pair(reinterpret_cast<const void*>(kernel), dyn_lds);
+    auto it = ctx->occupancy.find(key);
+    int per_cu;
+    if (it != ctx->occupancy.end()) {
+        per_cu = it->second;
+    } else {
+        per_cu = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, dyn_lds) !=
+                hipSuccess || per_cu < 1)
+            per_cu = 1;
+        ctx->occupancy[key] = per_cu;
+    }
+    const int64_t g = int64_t(per_cu) * std::max(ctx->num_cu, 1);
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, items));
+}
+
+// Fill a few words of device memory on the stream (a kernel, so it can be
 // captured in graphs).
-int set_u32(pic_context* ctx, uint32_t* p, uint32_t v);
+int fill_u32(pic_context* ctx, uint32_t* p, int n, uint32_t v);
 
 // Zero a grid window on the stream (used by PIC_P2G_ZERO_FIRST).
 int zero_grid(pic_context* ctx, int dtype, const pic_grid& g, int ncomp);

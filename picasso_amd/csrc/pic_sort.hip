@@ -18,7 +18,8 @@
 
 namespace pic {
 
-constexpr int SORT_LDS_CAP = 8192;     // particles per brick placed in LDS
+constexpr int SORT_LDS_CAP = 2048;     // particles per brick placed in LDS
+constexpr int SORT_BT = 256;           // work-items per brick-sort workgroup
 constexpr int SORT_MAX_BVOL = 4096;    // bins per brick
 constexpr int SCAN_THREADS = 256;
 constexpr int SCAN_ITEMS = 8;
@@ -160,7 +161,9 @@ __global__ __launch_bounds__(256) void k_bin_scatter(int64_t np, const uint32_t*
     if (ok) tmp[start[b] + r] = uint32_t(p);
 }
 
-// Sort one brick segment held in `items` (LDS or global scratch).
+// Sort one brick segment held in `items` (LDS or global scratch): count the
+// bins, scan, place each index in its bin (unordered), then give each index
+// its rank among the indices of its bin, which is its stable position.
 __device__ void brick_segment_sort(uint32_t b, uint32_t beg, uint32_t n, uint32_t bvol,
                                    const uint32_t* tmp, const uint32_t* keys_in,
                                    uint32_t* items, uint32_t* cnt, uint32_t* cur,
@@ -172,7 +175,7 @@ __device__ void brick_segment_sort(uint32_t b, uint32_t beg, uint32_t n, uint32_
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
         atomicAdd(&cnt[keys_in[tmp[beg + i]] - kb], 1u);
     __syncthreads();
-    // exclusive scan of cnt into cur (bvol <= SORT_MAX_BVOL; serial chunks)
+    // exclusive scan of cnt into cur (one wave, 64 bins at a time)
     if (threadIdx.x < 64) {
         const int lane = threadIdx.x;
         uint32_t carry = 0;
@@ -189,29 +192,23 @@ __device__ void brick_segment_sort(uint32_t b, uint32_t beg, uint32_t n, uint32_
         }
     }
     __syncthreads();
-    // place (unordered within a bin), remembering each bin's start in cnt
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         const uint32_t p = tmp[beg + i];
         const uint32_t pos = atomicAdd(&cur[keys_in[p] - kb], 1u);
         items[pos] = p;
     }
     __syncthreads();
-    // each bin: ascending indices (insertion sort; bins hold a few particles)
-    for (uint32_t l = threadIdx.x; l < bvol; l += blockDim.x) {
-        const uint32_t e = cur[l], c = cnt[l], s0 = e - c;
-        for (uint32_t i = s0 + 1; i < e; ++i) {
-            const uint32_t v = items[i];
-            uint32_t j = i;
-            while (j > s0 && items[j - 1] > v) {
-                items[j] = items[j - 1];
-                --j;
-            }
-            items[j] = v;
-        }
-        for (uint32_t i = s0; i < e; ++i) keys_out[beg + i] = kb + l;
+    // cur[l] is now the end of bin l.  Indices are distinct, so the ranks
+    // inside a bin are a permutation of its slots.
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t v = items[i];
+        const uint32_t l = keys_in[v] - kb;
+        const uint32_t e = cur[l];
+        uint32_t r = e - cnt[l];
+        for (uint32_t j = e - cnt[l]; j < e; ++j) r += items[j] < v ? 1u : 0u;
+        perm[beg + r] = v;
+        keys_out[beg + r] = kb + l;
     }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) perm[beg + i] = items[i];
 }
 
 // List of non-empty bricks: flags -> exclusive scan -> ids in increasing order.
@@ -230,19 +227,27 @@ __global__ __launch_bounds__(256) void k_compact_active(const uint32_t* flags, c
     if (b == nb) active[0] = pos[nb];
 }
 
-__global__ __launch_bounds__(256) void k_brick_sort(const uint32_t* start, const uint32_t* tmp,
-                                                    const uint32_t* keys_in, uint32_t bvol,
-                                                    uint32_t* big, uint32_t* perm,
-                                                    uint32_t* keys_out)
+// Workgroups walk the non-empty bricks with a grid stride.  Dynamic LDS:
+// cnt[bvol] | cur[bvol] | items[SORT_LDS_CAP]; larger bricks use `big`.
+__global__ __launch_bounds__(SORT_BT) void k_brick_sort(const uint32_t* start,
+                                                        const uint32_t* active,
+                                                        const uint32_t* tmp,
+                                                        const uint32_t* keys_in, uint32_t bvol,
+                                                        uint32_t* big, uint32_t* perm,
+                                                        uint32_t* keys_out)
 {
-    __shared__ uint32_t cnt[SORT_MAX_BVOL];
-    __shared__ uint32_t cur[SORT_MAX_BVOL];
-    __shared__ uint32_t lds_items[SORT_LDS_CAP];
-    const uint32_t b = blockIdx.x;
-    const uint32_t beg = start[b], n = start[b + 1] - beg;
-    if (n == 0) return;
-    uint32_t* items = n <= SORT_LDS_CAP ? lds_items : big + beg;
-    brick_segment_sort(b, beg, n, bvol, tmp, keys_in, items, cnt, cur, perm, keys_out);
+    extern __shared__ uint32_t sort_lds[];
+    uint32_t* cnt = sort_lds;
+    uint32_t* cur = sort_lds + bvol;
+    uint32_t* lds_items = sort_lds + 2 * bvol;
+    const uint32_t nact = active[0];
+    for (uint32_t t = blockIdx.x; t < nact; t += gridDim.x) {
+        const uint32_t b = active[1 + t];
+        const uint32_t beg = start[b], n = start[b + 1] - beg;
+        uint32_t* items = n <= SORT_LDS_CAP ? lds_items : big + beg;
+        brick_segment_sort(b, beg, n, bvol, tmp, keys_in, items, cnt, cur, perm, keys_out);
+        __syncthreads();
+    }
 }
 
 }  // namespace pic
@@ -299,10 +304,12 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
     DeviceGuard dg(ctx->device);
     if (!dg.ok) return PIC_ERR_HIP;
 
-    // scratch: count[nb+1] | cursor[nb] | scan tmp | keys_in[np] | tmp[w] | big[w]
-    // with w = max(np, nb+1) (tmp and big also hold the active-brick flags).
+    // scratch: count[nb+1] | cursor[nb] | scan tmp | act[nb+1] | keys_in[np] | tmp[w] | big[w]
+    // with w = max(np, nb+1) (tmp and big also hold the non-empty flags and
+    // their scan before the scatter).  act holds the active-brick list when
+    // the caller does not ask for it.
     const int64_t scan_words = scan_tmp_words(nb + 1);
-    const int64_t head = (nb + 1) + nb + scan_words;
+    const int64_t head = (nb + 1) + nb + scan_words + (active ? 0 : nb + 1);
     const int64_t w = std::max<int64_t>(np, nb + 1);
     const size_t bytes = size_t(head + np + 2 * w) * 4;
     uint32_t* sc = static_cast<uint32_t*>(scratch(ctx, bytes));
@@ -310,6 +317,7 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
     uint32_t* count = sc;
     uint32_t* cursor = count + nb + 1;
     uint32_t* scan_tmp = cursor + nb;
+    uint32_t* act = active ? active : scan_tmp + scan_words;
     uint32_t* keys_in = sc + head;
     uint32_t* tmp = keys_in + np;
     uint32_t* big = tmp + w;
@@ -328,20 +336,23 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
                                                    ctx->d_err);
     }
     exclusive_scan(st, count, bin_start, nb + 1, scan_tmp);
-    if (active) {
-        // flags and their scan live in tmp and big before the scatter uses them.
+    {
+        // non-empty brick list: flags and their scan live in tmp and big
+        // before the scatter uses them.
         uint32_t* flags = tmp;
         uint32_t* pos = big;
         const unsigned fb2 = blocks_for(nb + 1, 256);
         k_nonempty_flags<<<fb2, 256, 0, st>>>(count, nb, flags);
         exclusive_scan(st, flags, pos, nb + 1, scan_tmp);
-        k_compact_active<<<fb2, 256, 0, st>>>(flags, pos, nb, active);
+        k_compact_active<<<fb2, 256, 0, st>>>(flags, pos, nb, act);
     }
     if (np > 0) {
         k_bin_scatter<<<blocks_for(np, 256), 256, 0, st>>>(np, keys_in, g.bvol, bin_start, cursor,
                                                             tmp);
-        k_brick_sort<<<(unsigned)nb, 256, 0, st>>>(bin_start, tmp, keys_in, g.bvol, big, perm,
-                                                   keys);
+        const size_t lds = size_t(2 * g.bvol + SORT_LDS_CAP) * 4;
+        const unsigned nbk = persistent_blocks(ctx, k_brick_sort, SORT_BT, lds, nb);
+        k_brick_sort<<<nbk, SORT_BT, lds, st>>>(bin_start, act, tmp, keys_in, g.bvol, big, perm,
+                                                keys);
     }
     return launch_status();
 }
