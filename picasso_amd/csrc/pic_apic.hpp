@@ -131,6 +131,14 @@ __device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
 {
     constexpr int NK = Knots<ORDER>::NK;
     constexpr int NC = STAG ? 1 : 3;
+    // v_p = sum w u and B_p = sum w u d^T (Picasso_APIC.hpp:439-471), summed
+    // one dimension at a time: the k sums first (weights w_k and w_k d_k),
+    // then j, then i.
+    T wd[3][NK];
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int n = 0; n < NK; ++n) wd[e][n] = sd.w[e][n] * sd.d[e][n];
     T up[NC];
     T B[NC][3];
 #pragma unroll
@@ -139,25 +147,40 @@ __device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
         B[d][0] = B[d][1] = B[d][2] = T(0);
     }
 #pragma unroll
-    for (int i = 0; i < NK; ++i)
+    for (int i = 0; i < NK; ++i) {
+        T Si[NC], Sdi[NC], Ti[NC];
+#pragma unroll
+        for (int d = 0; d < NC; ++d) Si[d] = Sdi[d] = Ti[d] = T(0);
 #pragma unroll
         for (int j = 0; j < NK; ++j) {
-            const T wij = sd.w[0][i] * sd.w[1][j];
+            T sij[NC], tij[NC];
+#pragma unroll
+            for (int d = 0; d < NC; ++d) sij[d] = tij[d] = T(0);
 #pragma unroll
             for (int k = 0; k < NK; ++k) {
-                const T wip = wij * sd.w[2][k];
                 T ue[NC];
                 fetch(i, j, k, ue);
 #pragma unroll
                 for (int d = 0; d < NC; ++d) {
-                    const T wu = wip * ue[d];
-                    up[d] += wu;
-                    B[d][0] += wu * sd.d[0][i];
-                    B[d][1] += wu * sd.d[1][j];
-                    B[d][2] += wu * sd.d[2][k];
+                    sij[d] += sd.w[2][k] * ue[d];
+                    tij[d] += wd[2][k] * ue[d];
                 }
             }
+#pragma unroll
+            for (int d = 0; d < NC; ++d) {
+                Si[d] += sd.w[1][j] * sij[d];
+                Sdi[d] += wd[1][j] * sij[d];
+                Ti[d] += sd.w[1][j] * tij[d];
+            }
         }
+#pragma unroll
+        for (int d = 0; d < NC; ++d) {
+            up[d] += sd.w[0][i] * Si[d];
+            B[d][0] += wd[0][i] * Si[d];
+            B[d][1] += sd.w[0][i] * Sdi[d];
+            B[d][2] += sd.w[0][i] * Ti[d];
+        }
+    }
     if constexpr (STAG) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -246,6 +269,22 @@ __device__ __forceinline__ T node_velocity(const GView<T>& gu, const GView<T>& g
     if (!gm.data) return q;
     const T mm = gm.data[moff];
     return mm > T(0) ? T(double(q) / double(mm)) : T(0);
+}
+
+// All three components at a node, with one FP64 reciprocal of the mass.
+template <typename T>
+__device__ __forceinline__ void node_velocity3(const GView<T>& gu, const GView<T>& gm,
+                                               int64_t off, int64_t moff, T u[3])
+{
+    if (!gm.data) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) u[d] = gu.data[off + d * gu.cs];
+        return;
+    }
+    const T mm = gm.data[moff];
+    const double r = mm > T(0) ? 1.0 / double(mm) : 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = T(double(gu.data[off + d * gu.cs]) * r);
 }
 
 }  // namespace pic

@@ -28,8 +28,8 @@
 
 namespace pic {
 
-constexpr int BT = 256;  // work-items per brick workgroup
-constexpr int PB = 4;    // p2g brick edge in bins: 64 bins x 4 components = BT
+constexpr int BT = 256;  // work-items per g2p brick workgroup
+constexpr int PB = 4;    // p2g brick edge in bins
 
 template <typename T>
 struct BrickArgs {
@@ -99,11 +99,20 @@ __device__ __forceinline__ void push_stale(const BrickArgs<T>& g, uint32_t s)
     g.stale[1 + at] = s;
 }
 
-// LDS layout of the p2g bin sums: per component round, [component][stencil
-// offset][bin], each component padded by 16 words so the four components a
-// bin group stores at once start in different banks.  Components are staged
-// in 1, 2 or 4 rounds to keep the buffer near 34 KB (five workgroups per
-// compute unit).
+// Work split of the p2g brick kernel: four work-items per bin of a 4^3
+// brick, one per grid component (mass, momentum x/y/z), each walking every
+// particle of its bin.
+//
+// LDS, two uses of one buffer:
+//  - particle staging: the brick's particles are copied in chunks of CAP
+//    with coalesced loads (consecutive work-items, consecutive particles),
+//    as four 4-scalar groups per particle: (x0, x1, x2, m) and, per
+//    momentum component d, (u_d, B_d0, B_d1, B_d2).  A bin group then reads
+//    two groups per particle with two wide LDS loads instead of issuing
+//    eight gathers to memory.  Groups are XOR-swizzled by (q >> 2) & 3 so
+//    16 consecutive particles written at once hit distinct bank groups.
+//  - bin sums, after the last chunk: [component][stencil offset][bin], each
+//    component padded by 16 words; staged in rounds of CPR components.
 template <int ORDER, typename T>
 struct P2GShape {
     static constexpr int NK = ORDER + 1;
@@ -111,13 +120,26 @@ struct P2GShape {
     static constexpr int FN = PB + ORDER;
     static constexpr int FV = FN * FN * FN;
     static constexpr int BINS = PB * PB * PB;
+    static constexpr int NT = BINS * 4;  // work-items per workgroup
     static constexpr int CSTRIDE = NN * BINS + 16;
-    static constexpr int BUDGET = 34 * 1024;
-    static constexpr int NROUND = 4 * CSTRIDE * int(sizeof(T)) <= BUDGET   ? 1
-                                  : 2 * CSTRIDE * int(sizeof(T)) <= BUDGET ? 2
-                                                                            : 4;
+    static constexpr int SUM_BUDGET = 34 * 1024;
+    static constexpr int NROUND = 4 * CSTRIDE * int(sizeof(T)) <= SUM_BUDGET   ? 1
+                                  : 2 * CSTRIDE * int(sizeof(T)) <= SUM_BUDGET ? 2
+                                                                                : 4;
     static constexpr int CPR = 4 / NROUND;  // components per round
+    static constexpr int CAP = sizeof(T) == 4 ? 624 : 400;  // particles per chunk
+    static constexpr int STAGE = CAP * 16;                   // scalars
+    static constexpr int SUMS = CPR * CSTRIDE;
+    static constexpr int LDS = STAGE > SUMS ? STAGE : SUMS;
+    // Waves per SIMD the register allocation is held to.
+    static constexpr int WAVES = sizeof(T) == 8 && ORDER == 2 ? 3 : 1;
 };
+
+// Scalar offset of group g of staged particle q.
+__device__ __forceinline__ int stage_at(int q, int g)
+{
+    return q * 16 + 4 * (g ^ ((q >> 2) & 3));
+}
 
 // Accumulate one particle's component into the bin's stencil sums.  v and
 // B0..B2 are row `comp - 1` of u_p and B_p for a momentum component; the
@@ -173,21 +195,23 @@ __device__ __forceinline__ void p2g_component(const Spline<ORDER, T>& sd, T m, T
 }
 
 template <int ORDER, typename T>
-__device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b, T* sums,
+__device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b, T* lds,
                                               uint32_t* rng)
 {
     using S = P2GShape<ORDER, T>;
+    using T4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
     constexpr int NK = S::NK, NN = S::NN, FN = S::FN, FV = S::FV, BINS = S::BINS;
+    constexpr int NT = S::NT, CAP = S::CAP;
     const ApicArgs<T>& a = g.a;
     const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
     int f0[3];
     brick_origin<PB>(g, b, f0);
 
     // Bin ranges [rng[l], rng[BINS + l]) from the sorted keys (empty: 0, 0).
-    for (int l = threadIdx.x; l < 2 * BINS; l += BT) rng[l] = 0u;
+    for (int l = threadIdx.x; l < 2 * BINS; l += NT) rng[l] = 0u;
     __syncthreads();
     const uint32_t kb = b * BINS;
-    for (uint32_t s = beg + threadIdx.x; s < end; s += BT) {
+    for (uint32_t s = beg + threadIdx.x; s < end; s += NT) {
         const uint32_t key = g.keys[s];
         if (s == beg || g.keys[s - 1] != key) rng[key - kb] = s;
         if (s + 1 == end || g.keys[s + 1] != key) rng[BINS + key - kb] = s + 1;
@@ -197,44 +221,67 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
     const int l = int(threadIdx.x) >> 2, comp = int(threadIdx.x) & 3;
     const int li = l / (PB * PB), lj = (l / PB) % PB, lk = l % PB;
     const int e0 = f0[0] + li, e1 = f0[1] + lj, e2 = f0[2] + lk;
-    const int d = comp > 0 ? comp - 1 : 0;
+    const int grp = comp > 0 ? comp : 1;
     const T hh = (T)a.lat.h;
     const T Dinv = T(ORDER == 3 ? 3.0 : 4.0) / (hh * hh);  // APIC::inertialScaling (:33-47)
     T acc[NN];
 #pragma unroll
     for (int n = 0; n < NN; ++n) acc[n] = T(0);
-    const uint32_t s1 = rng[BINS + l];
-    for (uint32_t s = rng[l]; s < s1; ++s) {
-        const int64_t p = int64_t(a.perm[s]);
-        const int64_t xb = a.x.base(p);
-        const T xp[3] = {a.x.at(xb, 0), a.x.at(xb, 1), a.x.at(xb, 2)};
-        const T m = a.m.at(a.m.base(p), 0);
-        // B(d, q) = c_p(1 + q, d) (Picasso_APIC.hpp:94-101).
-        const int64_t cb = a.c.base(p);
-        T v = a.c.at(cb, d), B0 = a.c.at(cb, 3 + d), B1 = a.c.at(cb, 6 + d),
-          B2 = a.c.at(cb, 9 + d);
-        if (comp == 0) {
-            v = T(1);
-            B0 = B1 = B2 = T(0);
+    const uint32_t b0 = rng[l], b1 = rng[BINS + l];
+    for (uint32_t c0 = beg; c0 < end; c0 += CAP) {
+        const uint32_t c1 = min(end, c0 + uint32_t(CAP));
+        // Stage the chunk: consecutive work-items load consecutive particles.
+        for (uint32_t s = c0 + threadIdx.x; s < c1; s += NT) {
+            const int64_t p = int64_t(a.perm[s]);
+            const int64_t xb = a.x.base(p), cb = a.c.base(p);
+            const int q = int(s - c0);
+            T4 v;
+            v.x = a.x.at(xb, 0);
+            v.y = a.x.at(xb, 1);
+            v.z = a.x.at(xb, 2);
+            v.w = a.m.at(a.m.base(p), 0);
+            *reinterpret_cast<T4*>(lds + stage_at(q, 0)) = v;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                // B(d, a) = c_p(1 + a, d) (Picasso_APIC.hpp:94-101).
+                v.x = a.c.at(cb, d);
+                v.y = a.c.at(cb, 3 + d);
+                v.z = a.c.at(cb, 6 + d);
+                v.w = a.c.at(cb, 9 + d);
+                *reinterpret_cast<T4*>(lds + stage_at(q, 1 + d)) = v;
+            }
         }
-        Spline<ORDER, T> sd;
-        eval_spline<ORDER, T, ORDER == 1, ORDER != 1>(a.lat, xp, sd);
-        if (sd.s0[0] != e0 || sd.s0[1] != e1 || sd.s0[2] != e2) {
-            if (comp == 0) push_stale(g, s);
-            continue;
+        __syncthreads();
+        const uint32_t s1 = min(b1, c1);
+        for (uint32_t s = max(b0, c0); s < s1; ++s) {
+            const int q = int(s - c0);
+            const T4 xm = *reinterpret_cast<const T4*>(lds + stage_at(q, 0));
+            T4 ub = *reinterpret_cast<const T4*>(lds + stage_at(q, grp));
+            if (comp == 0) {
+                ub.x = T(1);
+                ub.y = ub.z = ub.w = T(0);
+            }
+            const T xp[3] = {xm.x, xm.y, xm.z};
+            Spline<ORDER, T> sd;
+            eval_spline<ORDER, T, ORDER == 1, ORDER != 1>(a.lat, xp, sd);
+            if (sd.s0[0] != e0 || sd.s0[1] != e1 || sd.s0[2] != e2) {
+                if (comp == 0) push_stale(g, s);
+                continue;
+            }
+            p2g_component<ORDER, T>(sd, xm.w, ub.x, ub.y, ub.z, ub.w, Dinv, acc);
         }
-        p2g_component<ORDER, T>(sd, m, v, B0, B1, B2, Dinv, acc);
+        __syncthreads();
     }
 
     for (int r = 0; r < S::NROUND; ++r) {
-        const int c_lo = r * S::CPR;
-        if (comp >= c_lo && comp < c_lo + S::CPR) {
-            T* dst = sums + (comp - c_lo) * S::CSTRIDE + l;
+        const int r_lo = r * S::CPR;
+        if (comp >= r_lo && comp < r_lo + S::CPR) {
+            T* dst = lds + (comp - r_lo) * S::CSTRIDE + l;
 #pragma unroll
             for (int n = 0; n < NN; ++n) dst[n * BINS] = acc[n];
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < FV; q += BT) {
+        for (int q = threadIdx.x; q < FV; q += NT) {
             const int fi = q / (FN * FN), fj = (q / FN) % FN, fk = q % FN;
             double tot[S::CPR];
 #pragma unroll
@@ -248,7 +295,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
                         const int bi = fi - oi, bj = fj - oj, bk = fk - ok;
                         if (bi < 0 || bi >= PB || bj < 0 || bj >= PB || bk < 0 || bk >= PB)
                             continue;
-                        const T* src = sums + ((oi * NK + oj) * NK + ok) * BINS +
+                        const T* src = lds + ((oi * NK + oj) * NK + ok) * BINS +
                                        (bi * PB + bj) * PB + bk;
 #pragma unroll
                         for (int cc = 0; cc < S::CPR; ++cc) tot[cc] += double(src[cc * S::CSTRIDE]);
@@ -265,7 +312,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
             const int64_t off = a.gmu.off(ni, nj, nk);
 #pragma unroll
             for (int cc = 0; cc < S::CPR; ++cc) {
-                const int c = c_lo + cc;
+                const int c = r_lo + cc;
                 const T v = T(tot[cc]);
                 if (v == T(0)) continue;
                 if (c == 0) atomic_add(a.gm.data + a.gm.off(ni, nj, nk), v);
@@ -277,13 +324,14 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
 }
 
 template <int ORDER, typename T>
-__global__ __launch_bounds__(BT) void k_apic_p2g_brick(BrickArgs<T> g)
+__global__ __launch_bounds__((P2GShape<ORDER, T>::NT), (P2GShape<ORDER, T>::WAVES)) void k_apic_p2g_brick(
+    BrickArgs<T> g)
 {
     using S = P2GShape<ORDER, T>;
-    __shared__ T sums[S::CPR * S::CSTRIDE];
+    __shared__ __align__(16) T lds[S::LDS];
     __shared__ uint32_t rng[2 * S::BINS];
     __shared__ uint32_t slot[2];
-    walk_bricks(g, slot, [&](uint32_t b) { p2g_one_brick<ORDER, T>(g, b, sums, rng); });
+    walk_bricks(g, slot, [&](uint32_t b) { p2g_one_brick<ORDER, T>(g, b, lds, rng); });
 }
 
 template <int BB, int ORDER, typename T>
@@ -301,9 +349,10 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, uint32_t b,
         const bool in = gu.covers(0, ni, ni) && gu.covers(1, nj, nj) && gu.covers(2, nk, nk);
         const int64_t off = in ? gu.off(ni, nj, nk) : 0;
         const int64_t moff = in && a.gnm.data ? a.gnm.off(ni, nj, nk) : 0;
+        T u[3] = {T(0), T(0), T(0)};
+        if (in) node_velocity3(gu, a.gnm, off, moff, u);
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-            us[d * FV + q] = in ? node_velocity(gu, a.gnm, off, moff, d) : T(0);
+        for (int d = 0; d < 3; ++d) us[d * FV + q] = u[d];
     }
     __syncthreads();
 
@@ -399,8 +448,9 @@ struct P2GBrickLaunch {
     template <int O>
     void go()
     {
-        const unsigned nbk = persistent_blocks(ctx, k_apic_p2g_brick<O, T>, BT, 0, g.nbricks);
-        k_apic_p2g_brick<O, T><<<nbk, BT, 0, ctx->stream>>>(g);
+        constexpr int NT = P2GShape<O, T>::NT;
+        const unsigned nbk = persistent_blocks(ctx, k_apic_p2g_brick<O, T>, NT, 0, g.nbricks);
+        k_apic_p2g_brick<O, T><<<nbk, NT, 0, ctx->stream>>>(g);
     }
     void by_order(int order)
     {

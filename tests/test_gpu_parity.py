@@ -184,24 +184,20 @@ def test_bin_sort_out_of_range_reported(pa, wl0):
         gu.ctx().synchronize()
 
 
-@pytest.mark.parametrize("order", [1, 2, 3])
-@pytest.mark.parametrize("entity", [0, 1])
-@pytest.mark.parametrize("dtype", ["f64", "f32"])
-@pytest.mark.parametrize("edge", [4, 8])
-def test_brick_transfers_vs_oracle(pa, wl0, order, entity, dtype, edge):
-    """Binned p2g/g2p (brick kernels) against the oracle."""
+def _check_brick(pa, wl, order, entity, dtype, edge):
+    """Binned p2g/g2p (brick kernels) on workload wl against the oracle."""
     import gpu_util as gu
     tdt = torch.float64 if dtype == "f64" else torch.float32
     f32 = dtype == "f32"
-    xh = wl0.x.astype(np.float32).astype(np.float64) if f32 else wl0.x
-    mh = wl0.m.astype(np.float32).astype(np.float64) if f32 else wl0.m
-    ch = wl0.c.astype(np.float32).astype(np.float64) if f32 else wl0.c
-    lo, n = wl0.node_window
-    mesh = pa.Mesh(wl0.low, wl0.h)
+    xh = wl.x.astype(np.float32).astype(np.float64) if f32 else wl.x
+    mh = wl.m.astype(np.float32).astype(np.float64) if f32 else wl.m
+    ch = wl.c.astype(np.float32).astype(np.float64) if f32 else wl.c
+    lo, n = wl.node_window
+    mesh = pa.Mesh(wl.low, wl.h)
     x = gu.soa(xh, tdt)
     m = gu.soa(mh[:, None], tdt)
     c = gu.soa(ch.reshape(-1, 12), tdt)
-    bins = pa.bin_sort(gu.ctx(), mesh, x, _binning(pa, wl0, order, entity, (edge,) * 3))
+    bins = pa.bin_sort(gu.ctx(), mesh, x, _binning(pa, wl, order, entity, (edge,) * 3))
     sd = pa.create_spline(entity, order, mesh, x, bins)
     gm, gmu = _node_grids(pa, lo, n, tdt)
     pa.APIC.p2g(gu.ctx(), m, c, gm, gmu, sd, zero_first=True)
@@ -214,13 +210,45 @@ def test_brick_transfers_vs_oracle(pa, wl0, order, entity, dtype, edge):
     gu.ctx().synchronize()
     rm = np.zeros(tuple(n))
     rmu = np.zeros(tuple(n) + (3,))
-    orpic.apic_p2g(order, entity, wl0.low, wl0.h, lo, n, xh, mh, ch, rm, rmu)
-    rc = orpic.apic_g2p(order, entity, wl0.low, wl0.h, lo, n, xh, gfield)
+    orpic.apic_p2g(order, entity, wl.low, wl.h, lo, n, xh, mh, ch, rm, rmu)
+    rc = orpic.apic_g2p(order, entity, wl.low, wl.h, lo, n, xh, gfield)
     tol = F32_TOL if f32 else F64_TOL
     assert gu.rel_inf(gu.host(gm.tensor), rm) < tol
     assert gu.rel_inf(gu.host(gmu.tensor), rmu) < tol
     assert gu.rel_inf(gu.host(cg).reshape(-1, 4, 3)[:, 0], rc[:, 0]) < tol
     assert gu.rel_inf(gu.host(cg).reshape(-1, 4, 3)[:, 1:], rc[:, 1:]) < tol * 10
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("entity", [0, 1])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("edge", [4, 8])
+def test_brick_transfers_vs_oracle(pa, wl0, order, entity, dtype, edge):
+    _check_brick(pa, wl0, order, entity, dtype, edge)
+
+
+@pytest.fixture(scope="module")
+def wl_dense():
+    """16^3 cells: 2000 particles spread over the box plus 6000 packed into
+    a 3-cell cube, so some bricks hold several staging chunks and some bins
+    hundreds of particles."""
+    from picasso_amd import synth
+    rng = np.random.default_rng(21)
+    n = 16
+    h = 1.0 / n
+    x = np.concatenate([rng.uniform(0, 1, size=(2000, 3)),
+                        0.5 + rng.uniform(-1.5 * h, 1.5 * h, size=(6000, 3))])
+    rng.shuffle(x)
+    m = rng.uniform(0.5, 1.5, len(x)) * h ** 3
+    c = rng.normal(0, 0.1, size=(len(x), 4, 3))
+    c[:, 0] = rng.normal(size=(len(x), 3))
+    return synth.Workload("dense_cluster_16^3", n, 0, h, np.full(3, -synth.HALO * h), x, m, c=c)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_brick_dense_cluster(pa, wl_dense, order, dtype):
+    _check_brick(pa, wl_dense, order, 0, dtype, 4)
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
