@@ -106,11 +106,14 @@ __device__ __forceinline__ void push_stale(const BrickArgs<T>& g, uint32_t s)
 // LDS, two uses of one buffer:
 //  - particle staging: the brick's particles are copied in chunks of CAP
 //    with coalesced loads (consecutive work-items, consecutive particles),
-//    as four 4-scalar groups per particle: (x0, x1, x2, m) and, per
-//    momentum component d, (u_d, B_d0, B_d1, B_d2).  A bin group then reads
-//    two groups per particle with two wide LDS loads instead of issuing
-//    eight gathers to memory.  Groups are XOR-swizzled by (q >> 2) & 3 so
-//    16 consecutive particles written at once hit distinct bank groups.
+//    as four 4-scalar groups per particle: (f0, f1, f2, m) and, per
+//    momentum component d, (u_d, B_d0, B_d1, B_d2), where f is the
+//    fractional knot coordinate (knot base found once per particle here, in
+//    FP64, and checked against the bin; a mismatch is marked with f0 =
+//    STALE_F).  A bin group then reads two groups per particle with two
+//    wide LDS loads instead of issuing eight gathers to memory.  Groups are
+//    XOR-swizzled by (q >> 2) & 3 so 16 consecutive particles written at
+//    once hit distinct bank groups.
 //  - bin sums, after the last chunk: [component][stencil offset][bin], each
 //    component padded by 16 words; staged in rounds of CPR components.
 template <int ORDER, typename T>
@@ -134,6 +137,8 @@ struct P2GShape {
     // Waves per SIMD the register allocation is held to.
     static constexpr int WAVES = sizeof(T) == 8 && ORDER == 2 ? 3 : 1;
 };
+
+constexpr double STALE_F = 4.0;  // staged f0 of a particle the brick skips
 
 // Scalar offset of group g of staged particle q.
 __device__ __forceinline__ int stage_at(int q, int g)
@@ -219,8 +224,6 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
     __syncthreads();
 
     const int l = int(threadIdx.x) >> 2, comp = int(threadIdx.x) & 3;
-    const int li = l / (PB * PB), lj = (l / PB) % PB, lk = l % PB;
-    const int e0 = f0[0] + li, e1 = f0[1] + lj, e2 = f0[2] + lk;
     const int grp = comp > 0 ? comp : 1;
     const T hh = (T)a.lat.h;
     const T Dinv = T(ORDER == 3 ? 3.0 : 4.0) / (hh * hh);  // APIC::inertialScaling (:33-47)
@@ -235,10 +238,23 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
             const int64_t p = int64_t(a.perm[s]);
             const int64_t xb = a.x.base(p), cb = a.c.base(p);
             const int q = int(s - c0);
+            const int lb = int(g.keys[s] - kb);
+            const int eb[3] = {f0[0] + lb / (PB * PB), f0[1] + (lb / PB) % PB, f0[2] + lb % PB};
             T4 v;
-            v.x = a.x.at(xb, 0);
-            v.y = a.x.at(xb, 1);
-            v.z = a.x.at(xb, 2);
+            T* fr = &v.x;
+            bool ok = true;
+#pragma unroll
+            for (int dim = 0; dim < 3; ++dim) {
+                double fd;
+                const double kb0 = knot_base<ORDER>(double(a.x.at(xb, dim)), a.lat.o[dim],
+                                                     a.lat.rdx, fd);
+                ok = ok && int(kb0) + Knots<ORDER>::FIRST == eb[dim];
+                fr[dim] = T(fd);
+            }
+            if (!ok) {
+                push_stale(g, s);
+                v.x = T(STALE_F);
+            }
             v.w = a.m.at(a.m.base(p), 0);
             *reinterpret_cast<T4*>(lds + stage_at(q, 0)) = v;
 #pragma unroll
@@ -255,20 +271,17 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
         const uint32_t s1 = min(b1, c1);
         for (uint32_t s = max(b0, c0); s < s1; ++s) {
             const int q = int(s - c0);
-            const T4 xm = *reinterpret_cast<const T4*>(lds + stage_at(q, 0));
+            const T4 fm = *reinterpret_cast<const T4*>(lds + stage_at(q, 0));
+            if (fm.x == T(STALE_F)) continue;
             T4 ub = *reinterpret_cast<const T4*>(lds + stage_at(q, grp));
             if (comp == 0) {
                 ub.x = T(1);
                 ub.y = ub.z = ub.w = T(0);
             }
-            const T xp[3] = {xm.x, xm.y, xm.z};
+            const T fr[3] = {fm.x, fm.y, fm.z};
             Spline<ORDER, T> sd;
-            eval_spline<ORDER, T, ORDER == 1, ORDER != 1>(a.lat, xp, sd);
-            if (sd.s0[0] != e0 || sd.s0[1] != e1 || sd.s0[2] != e2) {
-                if (comp == 0) push_stale(g, s);
-                continue;
-            }
-            p2g_component<ORDER, T>(sd, xm.w, ub.x, ub.y, ub.z, ub.w, Dinv, acc);
+            spline_from_frac<ORDER, T, ORDER == 1, ORDER != 1>(a.lat, fr, sd);
+            p2g_component<ORDER, T>(sd, fm.w, ub.x, ub.y, ub.z, ub.w, Dinv, acc);
         }
         __syncthreads();
     }
@@ -356,28 +369,47 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, uint32_t b,
     }
     __syncthreads();
 
-    for (uint32_t s = beg + threadIdx.x; s < end; s += BT) {
-        const int64_t p = int64_t(a.perm[s]);
-        const int64_t xb = a.x.base(p);
-        T xp[3] = {a.x.at(xb, 0), a.x.at(xb, 1), a.x.at(xb, 2)};
-        Spline<ORDER, T> sd;
-        eval_spline<ORDER, T, false, true>(a.lat, xp, sd);
-        const int li = sd.s0[0] - f0[0], lj = sd.s0[1] - f0[1], lk = sd.s0[2] - f0[2];
-        if (!(li >= 0 && li < BB && lj >= 0 && lj < BB && lk >= 0 && lk < BB)) {
-            push_stale(g, s);
-            continue;
-        }
-        const int lbase = (li * FN + lj) * FN + lk;
-        T cp[12];
-        apic_g2p_particle<ORDER, false>(a, sd, [&](int i, int j, int k, T* ue) {
-            const int q = lbase + (i * FN + j) * FN + k;
-            ue[0] = us[q];
-            ue[1] = us[FV + q];
-            ue[2] = us[2 * FV + q];
-        }, cp);
-        const int64_t cb = a.c.base(p);
+    // Up to R particles per work-item at a time, with all their position
+    // loads issued before any result is stored.
+    constexpr int R = 3;
+    for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += R * BT) {
+        int64_t pp[R];
+        T xp[R][3];
 #pragma unroll
-        for (int r = 0; r < 12; ++r) a.c.at(cb, r) = cp[r];
+        for (int r = 0; r < R; ++r) {
+            const uint32_t s = s0 + uint32_t(r * BT);
+            pp[r] = s < end ? int64_t(a.perm[s]) : 0;
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if (s0 + uint32_t(r * BT) >= end) continue;
+            const int64_t xb = a.x.base(pp[r]);
+#pragma unroll
+            for (int dim = 0; dim < 3; ++dim) xp[r][dim] = a.x.at(xb, dim);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t s = s0 + uint32_t(r * BT);
+            if (s >= end) continue;
+            Spline<ORDER, T> sd;
+            eval_spline<ORDER, T, false, true>(a.lat, xp[r], sd);
+            const int li = sd.s0[0] - f0[0], lj = sd.s0[1] - f0[1], lk = sd.s0[2] - f0[2];
+            if (!(li >= 0 && li < BB && lj >= 0 && lj < BB && lk >= 0 && lk < BB)) {
+                push_stale(g, s);
+                continue;
+            }
+            const int lbase = (li * FN + lj) * FN + lk;
+            T cp[12];
+            apic_g2p_particle<ORDER, false>(a, sd, [&](int i, int j, int k, T* ue) {
+                const int q = lbase + (i * FN + j) * FN + k;
+                ue[0] = us[q];
+                ue[1] = us[FV + q];
+                ue[2] = us[2 * FV + q];
+            }, cp);
+            const int64_t cb = a.c.base(pp[r]);
+#pragma unroll
+            for (int e = 0; e < 12; ++e) a.c.at(cb, e) = cp[e];
+        }
     }
     __syncthreads();
 }

@@ -267,6 +267,32 @@ __device__ __forceinline__ void eval_spline(const Lattice& lat, const T xp[3],
     }
 }
 
+// Spline weights (and physical gradients / distances) from the fractional
+// knot coordinate f = t - base of each dimension, as eval_spline forms them
+// (the FP64 distances come from f here, not from the absolute positions).
+template <int ORDER, typename T, bool GRAD, bool DIST>
+__device__ __forceinline__ void spline_from_frac(const Lattice& lat, const T f[3],
+                                                 Spline<ORDER, T>& sd)
+{
+    constexpr int NK = Knots<ORDER>::NK;
+    constexpr int FIRST = Knots<ORDER>::FIRST;
+#pragma unroll
+    for (int dim = 0; dim < 3; ++dim) {
+        bspline_w<ORDER, T>(f[dim], sd.w[dim]);
+        if constexpr (GRAD) {
+            bspline_g<ORDER, T>(f[dim], sd.g[dim]);
+            const T r = (T)lat.rdx;
+#pragma unroll
+            for (int n = 0; n < NK; ++n) sd.g[dim][n] *= r;
+        }
+        if constexpr (DIST) {
+            const T hh = (T)lat.h;
+#pragma unroll
+            for (int n = 0; n < NK; ++n) sd.d[dim][n] = hh * (T(FIRST + n) - f[dim]);
+        }
+    }
+}
+
 // Is the whole stencil inside the grid window?
 template <int ORDER, typename T, typename G>
 __device__ __forceinline__ bool stencil_inside(const Spline<ORDER, T>& sd,
