@@ -71,24 +71,31 @@ __device__ __forceinline__ uint32_t walk_brick(const BrickArgs<T>& g, uint32_t t
 }
 
 // Drive body(brick) over the walk.  A workgroup's first index is its
-// blockIdx; each later one is requested from the counter when the previous
-// brick starts, so the request's latency hides behind that brick.  Every
-// workgroup ends after drawing an index past the end.  `slot` is a two-word
-// LDS buffer (alternating, so no extra barrier is needed when a brick is
-// skipped).
-template <typename T, typename Body>
+// blockIdx.  With DYN, each later one is requested from the counter when the
+// previous brick starts (the request's latency hides behind that brick), so
+// bricks start in list order whatever each one costs; every workgroup ends
+// after drawing an index past the end.  `slot` is a two-word LDS buffer
+// (alternating, so no extra barrier is needed when a brick is skipped).
+// Without DYN the walk is a plain grid stride, which suits kernels whose
+// bricks cost about the same (a contended counter then only adds latency).
+template <bool DYN, typename T, typename Body>
 __device__ __forceinline__ void walk_bricks(const BrickArgs<T>& g, uint32_t* slot, Body&& body)
 {
     const uint32_t nw = walk_count(g);
     uint32_t t = blockIdx.x;
     for (int it = 0; t < nw; ++it) {
-        uint32_t nxt = 0;
-        if (threadIdx.x == 0) nxt = gridDim.x + atomicAdd(g.walk, 1u);
+        uint32_t nxt = t + gridDim.x;
+        if (DYN && threadIdx.x == 0) nxt = gridDim.x + atomicAdd(g.walk, 1u);
         const uint32_t b = walk_brick(g, t);
         if (g.bin_start[b] != g.bin_start[b + 1]) body(b);
-        if (threadIdx.x == 0) slot[it & 1] = nxt;
-        __syncthreads();
-        t = slot[it & 1];
+        if constexpr (DYN) {
+            if (threadIdx.x == 0) slot[it & 1] = nxt;
+            __syncthreads();
+            t = slot[it & 1];
+        } else {
+            __syncthreads();
+            t = nxt;
+        }
     }
 }
 
@@ -344,7 +351,7 @@ __global__ __launch_bounds__((P2GShape<ORDER, T>::NT), (P2GShape<ORDER, T>::WAVE
     __shared__ __align__(16) T lds[S::LDS];
     __shared__ uint32_t rng[2 * S::BINS];
     __shared__ uint32_t slot[2];
-    walk_bricks(g, slot, [&](uint32_t b) { p2g_one_brick<ORDER, T>(g, b, lds, rng); });
+    walk_bricks<true>(g, slot, [&](uint32_t b) { p2g_one_brick<ORDER, T>(g, b, lds, rng); });
 }
 
 template <int BB, int ORDER, typename T>
@@ -420,7 +427,7 @@ __global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
     constexpr int FN = BB + ORDER;
     __shared__ T us[3 * FN * FN * FN];
     __shared__ uint32_t slot[2];
-    walk_bricks(g, slot, [&](uint32_t b) { g2p_one_brick<BB, ORDER, T>(g, b, us); });
+    walk_bricks<false>(g, slot, [&](uint32_t b) { g2p_one_brick<BB, ORDER, T>(g, b, us); });
 }
 
 // Knot offset of the bins for (order, entity), or false when the bin
