@@ -26,8 +26,16 @@
  *    Picasso_PolyPIC.hpp:260), except a staggered PolyPIC g2p, which only
  *    rewrites the column it works on (Picasso_PolyPIC.hpp:328).
  *  - dtype PIC_F64: every field is double.  PIC_F32: every field is float;
- *    the position->cell mapping runs in FP64, p2g through bins accumulates
- *    with compensated (two-sum) FP32 sums.
+ *    the position->knot mapping runs in FP64.  p2g through bins sums each
+ *    bin's contributions in FP32 registers, adds the bin sums that land on a
+ *    node in FP64, and applies one FP32 atomic per node and brick.
+ *  - Brick kernels: p2g and g2p take the brick path when bins are given, the
+ *    binning is aligned with the spline's knot base (dual = 1 for quadratic
+ *    Node or linear/cubic Cell, dual = 0 for linear/cubic Node or quadratic
+ *    Cell) and bricks are 4x4x4 (p2g) or 4x4x4 / 8x8x8 (g2p).  Otherwise the
+ *    per-particle atomic scatter / gather kernels run.  Particles whose
+ *    position no longer matches their bin go through the per-particle path,
+ *    so results never depend on the bins being current.
  */
 #ifndef PICASSO_AMD_H
 #define PICASSO_AMD_H

@@ -1,0 +1,153 @@
+// picasso_amd.hpp -- header-only C++ facade over the C ABI in picasso_amd.h
+// that keeps Picasso's argument order for the transfer path:
+//
+//   Picasso::APIC::p2g(m_p, c_p, m_i, mu_i, sd)          (Picasso_APIC.hpp:55-65)
+//   Picasso::APIC::g2p(u_i, c_p, sd)                     (Picasso_APIC.hpp:407-413)
+//   Picasso::PolyPIC::p2g(m_p, u_p, c_p, mu_i, m_i, dt, sd) (Picasso_PolyPIC.hpp:39-50)
+//   Picasso::PolyPIC::g2p(u_i, c_p, sd)                  (Picasso_PolyPIC.hpp:235-243)
+//
+// The reference calls these once per particle inside a Kokkos kernel; here
+// each call covers every particle of the spline data at once (the batch form
+// the C ABI exports).  `sd` is the analogue of createSpline(Location,
+// InterpolationOrder, local_mesh, x) (Picasso_ParticleInterpolation.hpp:
+// 88-105): it records where the positions live, the spline order and the
+// entity, and the kernels evaluate the weights themselves.  Errors are
+// thrown as picasso_amd::Error carrying the pic_status; device-side errors
+// surface at Context::synchronize().
+#ifndef PICASSO_AMD_HPP
+#define PICASSO_AMD_HPP
+
+#include <stdexcept>
+#include <string>
+
+#include "picasso_amd.h"
+
+namespace picasso_amd {
+
+struct Error : std::runtime_error {
+    int status;
+    explicit Error(int s)
+        : std::runtime_error(std::string("picasso_amd: ") + pic_status_string(s)), status(s)
+    {
+    }
+};
+
+inline void check(int status)
+{
+    if (status != PIC_OK) throw Error(status);
+}
+
+// Owns a pic_context (device, stream binding, error word, scratch).
+class Context {
+  public:
+    explicit Context(int device = 0) { check(pic_context_create(&ctx_, device)); }
+    ~Context()
+    {
+        if (ctx_) pic_context_destroy(ctx_);
+    }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+    void set_stream(void* hip_stream) { check(pic_set_stream(ctx_, hip_stream)); }
+    void synchronize() { check(pic_synchronize(ctx_)); }
+    pic_context* get() const { return ctx_; }
+
+  private:
+    pic_context* ctx_ = nullptr;
+};
+
+// Particle field descriptors (see pic_pfield).
+inline pic_pfield soa(void* data, int64_t np) { return pic_pfield{data, 0, 1, np}; }
+inline pic_pfield aos(void* data, int ncomp) { return pic_pfield{data, 0, ncomp, 1}; }
+// Cabana AoSoA slice: vector length 2^vlen_log2, tile of tile_stride
+// scalars, data pointing at the member's first scalar
+// (Picasso_ParticleList.hpp:90-138).
+inline pic_pfield aosoa(void* data, int vlen_log2, int64_t tile_stride)
+{
+    return pic_pfield{data, vlen_log2, tile_stride, int64_t(1) << vlen_log2};
+}
+
+// Dense grid window [lo, lo+n) with the component index fastest.
+inline pic_grid dense_grid(void* data, const int lo[3], const int n[3], int ncomp)
+{
+    pic_grid g;
+    g.data = data;
+    for (int d = 0; d < 3; ++d) {
+        g.lo[d] = lo[d];
+        g.n[d] = n[d];
+    }
+    g.cstride = 1;
+    g.stride[2] = ncomp;
+    g.stride[1] = int64_t(n[2]) * ncomp;
+    g.stride[0] = int64_t(n[1]) * n[2] * ncomp;
+    return g;
+}
+
+// Batch spline data: createSpline(Location, InterpolationOrder<order>,
+// local_mesh, x) for particles [0, np).  bins (from pic_bin_sort) selects
+// the brick kernels when the binning matches the spline's knot lattice.
+struct SplineData {
+    int order;
+    int entity;
+    int dtype;
+    pic_mesh mesh;
+    int64_t np;
+    pic_pfield x;
+    const pic_bins* bins;
+};
+
+inline SplineData createSpline(int entity, int order, int dtype, const pic_mesh& mesh,
+                               int64_t np, const pic_pfield& x,
+                               const pic_bins* bins = nullptr)
+{
+    return SplineData{order, entity, dtype, mesh, np, x, bins};
+}
+
+namespace APIC {
+
+inline void p2g(pic_context* ctx, const pic_pfield& m_p, const pic_pfield& c_p,
+                const pic_grid& m_i, const pic_grid& mu_i, const SplineData& sd,
+                uint32_t flags = 0)
+{
+    check(pic_apic_p2g(ctx, sd.order, sd.entity, sd.dtype, &sd.mesh, sd.np, &sd.x, &m_p, &c_p,
+                       &m_i, &mu_i, sd.bins, flags));
+}
+
+inline void g2p(pic_context* ctx, const pic_grid& u_i, const pic_pfield& c_p,
+                const SplineData& sd)
+{
+    check(pic_apic_g2p(ctx, sd.order, sd.entity, sd.dtype, &sd.mesh, sd.np, &sd.x, &u_i, &c_p,
+                       sd.bins));
+}
+
+// g2p of u_i = mu_i / m_i formed on the fly (grid normalisation fused in).
+inline void g2p_normalized(pic_context* ctx, const pic_grid& m_i, const pic_grid& mu_i,
+                           const pic_pfield& c_p, const SplineData& sd)
+{
+    check(pic_apic_g2p_normalized(ctx, sd.order, sd.entity, sd.dtype, &sd.mesh, sd.np, &sd.x,
+                                  &m_i, &mu_i, &c_p, sd.bins));
+}
+
+}  // namespace APIC
+
+namespace PolyPIC {
+
+inline void p2g(pic_context* ctx, const pic_pfield& m_p, const pic_pfield& u_p,
+                const pic_pfield& c_p, const pic_grid& mu_i, const pic_grid& m_i, double dt,
+                const SplineData& sd, int ncomp = 3, uint32_t flags = 0)
+{
+    check(pic_polypic_p2g(ctx, sd.order, sd.entity, sd.dtype, &sd.mesh, sd.np, ncomp, &sd.x,
+                          &m_p, &u_p, &c_p, &mu_i, &m_i, dt, sd.bins, flags));
+}
+
+inline void g2p(pic_context* ctx, const pic_grid& u_i, const pic_pfield& c_p,
+                const SplineData& sd, int ncomp = 3)
+{
+    check(pic_polypic_g2p(ctx, sd.order, sd.entity, sd.dtype, &sd.mesh, sd.np, ncomp, &sd.x,
+                          &u_i, &c_p, sd.bins));
+}
+
+}  // namespace PolyPIC
+
+}  // namespace picasso_amd
+
+#endif

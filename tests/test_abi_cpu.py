@@ -74,3 +74,46 @@ def test_no_fallback_when_library_missing(tmp_path, monkeypatch):
     monkeypatch.setattr(os.path, "exists", lambda p: False if p.endswith(".so") else True)
     with pytest.raises(ImportError):
         spec.loader.exec_module(mod)
+
+
+CPP_PROBE = r"""
+#include "picasso_amd.hpp"
+#include <cstdio>
+int main() {
+    namespace pa = picasso_amd;
+    pic_binning g{{0, 0, 0}, {10, 10, 10}, {4, 4, 4}, 0};
+    pic_mesh mesh{{0.0, 0.0, 0.0}, 0.5};
+    double buf[12];
+    pic_pfield x = pa::soa(buf, 4);
+    pa::SplineData sd = pa::createSpline(PIC_NODE, 2, PIC_F64, mesh, 4, x);
+    int lo[3] = {0, 0, 0}, n[3] = {4, 4, 4};
+    pic_grid gm = pa::dense_grid(buf, lo, n, 1);
+    try {
+        pa::APIC::p2g(nullptr, x, x, gm, gm, sd);  // null context: argument error
+        return 2;
+    } catch (const pa::Error& e) {
+        std::printf("%d %lld %d\n", e.status, (long long)pic_bin_count(&g), pic_api_version());
+    }
+    return 0;
+}
+"""
+
+
+def test_cpp_facade_compiles_links_and_reports_errors(tmp_path):
+    """The C++ facade builds with a plain host compiler against the library
+    and turns a pic_status into an exception (no GPU call is made)."""
+    import shutil
+    import subprocess
+    from picasso_amd import _abi
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("no g++")
+    src = tmp_path / "probe.cpp"
+    src.write_text(CPP_PROBE)
+    exe = tmp_path / "probe"
+    libdir = os.path.dirname(_abi.LIB_PATH)
+    subprocess.run([gxx, "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"),
+                    str(src), "-o", str(exe), "-L", libdir, "-lpicasso_amd",
+                    "-Wl,-rpath," + libdir], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()
+    assert out == [str(_abi.PIC_ERR_ARG), "27", "1"]
