@@ -30,6 +30,8 @@ METRIC = ("APIC p2g+g2p particles/s @256³×8ppc, FP32-comp vs FP64 at stated er
           "HBM GB/s")
 HBM_PEAK_GBS = 8000.0
 BRICK = (4, 4, 4)
+CALIB_BYTES = 256 << 20  # device copy read once and written once
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic.json")
 
 
 def log(msg):
@@ -167,6 +169,13 @@ def main():
     log(f"touched nodes {n_touched}")
 
     ctx = pa.Context(0)
+    # One plain device copy of known size: under rocprofv3 --pmc its
+    # FETCH_SIZE / WRITE_SIZE calibrate the counters against known bytes.
+    calib = torch.empty(CALIB_BYTES // 4, dtype=torch.float32, device="cuda:0").fill_(1.0)
+    calib_dst = torch.empty_like(calib)
+    calib_dst.copy_(calib)
+    torch.cuda.synchronize()
+    del calib, calib_dst
     x32, m32, c32 = wl.x.astype(np.float32), wl.m.astype(np.float32), wl.c.astype(np.float32)
     families = ["bin_sort", "zero_grid", "apic_p2g", "max_mass", "normalize", "apic_g2p"]
 
@@ -223,11 +232,20 @@ def main():
     }
     dom = max(kt32, key=lambda f: kt32[f]["ms_total"])
     roof = None
+    # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+    # on this workload, scaled by the calibration copy (see the .md it names).
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get(dom)
     if dom in per_kernel_bytes:
         gbs = per_kernel_bytes[dom] / (kt32[dom]["ms_avg"] * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": dom, "achieved": gbs, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
                 "alg_bytes_per_launch": per_kernel_bytes[dom]}
+        if traffic:
+            roof["traffic_source"] = traffic["source"]
     else:
         roof = {"bound": "hbm", "kernel": dom, "achieved": None, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": None, "traffic": None,
