@@ -121,8 +121,10 @@ __device__ __forceinline__ void push_stale(const BrickArgs<T>& g, uint32_t s)
 //    wide LDS loads instead of issuing eight gathers to memory.  Groups are
 //    XOR-swizzled by (q >> 2) & 3 so 16 consecutive particles written at
 //    once hit distinct bank groups.
-//  - bin sums, after the last chunk: [component][stencil offset][bin], each
-//    component padded by 16 words; staged in rounds of CPR components.
+//  - bin sums, after the last chunk: [stencil offset][bin][component], the
+//    CPR components of a round contiguous, so a footprint node reads each
+//    (offset, bin) slot with one wide LDS load; staged in rounds of CPR
+//    components to bound the buffer.
 template <int ORDER, typename T>
 struct P2GShape {
     static constexpr int NK = ORDER + 1;
@@ -131,21 +133,27 @@ struct P2GShape {
     static constexpr int FV = FN * FN * FN;
     static constexpr int BINS = PB * PB * PB;
     static constexpr int NT = BINS * 4;  // work-items per workgroup
-    static constexpr int CSTRIDE = NN * BINS + 16;
+    static constexpr int SLOTS = NN * BINS;  // (offset, bin) slots
     static constexpr int SUM_BUDGET = 34 * 1024;
-    static constexpr int NROUND = 4 * CSTRIDE * int(sizeof(T)) <= SUM_BUDGET   ? 1
-                                  : 2 * CSTRIDE * int(sizeof(T)) <= SUM_BUDGET ? 2
-                                                                                : 4;
+    static constexpr int NROUND = 4 * SLOTS * int(sizeof(T)) <= SUM_BUDGET   ? 1
+                                  : 2 * SLOTS * int(sizeof(T)) <= SUM_BUDGET ? 2
+                                                                              : 4;
     static constexpr int CPR = 4 / NROUND;  // components per round
     static constexpr int CAP = sizeof(T) == 4 ? 624 : 400;  // particles per chunk
     static constexpr int STAGE = CAP * 16;                   // scalars
-    static constexpr int SUMS = CPR * CSTRIDE;
+    static constexpr int SUMS = CPR * SLOTS;
     static constexpr int LDS = STAGE > SUMS ? STAGE : SUMS;
     // Waves per SIMD the register allocation is held to.
     static constexpr int WAVES = sizeof(T) == 8 && ORDER == 2 ? 3 : 1;
 };
 
 constexpr double STALE_F = 4.0;  // staged f0 of a particle the brick skips
+
+// The components of one bin-sum slot, read with one wide LDS load.
+template <typename T, int N>
+struct alignas(sizeof(T) * N) SumSlot {
+    T c[N];
+};
 
 // Scalar offset of group g of staged particle q.
 __device__ __forceinline__ int stage_at(int q, int g)
@@ -296,9 +304,9 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
     for (int r = 0; r < S::NROUND; ++r) {
         const int r_lo = r * S::CPR;
         if (comp >= r_lo && comp < r_lo + S::CPR) {
-            T* dst = lds + (comp - r_lo) * S::CSTRIDE + l;
+            T* dst = lds + l * S::CPR + (comp - r_lo);
 #pragma unroll
-            for (int n = 0; n < NN; ++n) dst[n * BINS] = acc[n];
+            for (int n = 0; n < NN; ++n) dst[n * BINS * S::CPR] = acc[n];
         }
         __syncthreads();
         for (int q = threadIdx.x; q < FV; q += NT) {
@@ -306,7 +314,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
             double tot[S::CPR];
 #pragma unroll
             for (int cc = 0; cc < S::CPR; ++cc) tot[cc] = 0.0;
-#pragma unroll
+#pragma unroll 1
             for (int oi = 0; oi < NK; ++oi)
 #pragma unroll
                 for (int oj = 0; oj < NK; ++oj)
@@ -315,10 +323,11 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
                         const int bi = fi - oi, bj = fj - oj, bk = fk - ok;
                         if (bi < 0 || bi >= PB || bj < 0 || bj >= PB || bk < 0 || bk >= PB)
                             continue;
-                        const T* src = lds + ((oi * NK + oj) * NK + ok) * BINS +
-                                       (bi * PB + bj) * PB + bk;
+                        const int slot = ((oi * NK + oj) * NK + ok) * BINS + (bi * PB + bj) * PB + bk;
+                        const SumSlot<T, S::CPR> v =
+                            *reinterpret_cast<const SumSlot<T, S::CPR>*>(lds + slot * S::CPR);
 #pragma unroll
-                        for (int cc = 0; cc < S::CPR; ++cc) tot[cc] += double(src[cc * S::CSTRIDE]);
+                        for (int cc = 0; cc < S::CPR; ++cc) tot[cc] += double(v.c[cc]);
                     }
             bool any = false;
 #pragma unroll
