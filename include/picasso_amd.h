@@ -179,6 +179,22 @@ int pic_apic_g2p_normalized(pic_context* ctx, int order, int entity,
                             const pic_grid* mu_i, const pic_pfield* c_p,
                             const pic_bins* bins);
 
+/* APIC::g2p (Picasso_APIC.hpp:407-472) fused with the velocity gradient
+ * L = sum_n u_n (x) grad w_n (G2P::gradient, Picasso_ParticleInterpolation.hpp:
+ * 165-181; L(a,b) = du_a/dx_b) and the deformation-gradient update
+ * F_p <- (I + dt L) F_p, det(F_p), F_p^{-1} in FP64 (Picasso_BatchedLinearAlgebra.hpp:
+ * 3806-3834 product, :4335-4350 det, :4624-4685 inverse), in one pass over
+ * the particles.  Node/Cell entities.  m_i NULL: u_i is the grid velocity;
+ * otherwise u_i is the grid momentum and the velocity mu_i / m_i is formed
+ * on the fly as in pic_apic_g2p_normalized.  F, det, Finv are double (9, 1,
+ * 9 components); F is updated in place. */
+int pic_apic_g2p_update_F(pic_context* ctx, int order, int entity, int dtype,
+                          const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                          const pic_grid* m_i, const pic_grid* u_i,
+                          const pic_pfield* c_p, double dt, const pic_pfield* F,
+                          const pic_pfield* det, const pic_pfield* Finv,
+                          const pic_bins* bins);
+
 /* Number of PolyPIC modes per component for a spline order: 8 (linear, the
  * reference's), 27 (orders 2 and 3, the {1,y,y^2}^3 basis). */
 int pic_polypic_nmode(int order);

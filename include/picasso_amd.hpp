@@ -127,6 +127,16 @@ inline void g2p_normalized(pic_context* ctx, const pic_grid& m_i, const pic_grid
                                   &m_i, &mu_i, &c_p, sd.bins));
 }
 
+// g2p fused with the velocity gradient and the FP64 update of F (and its
+// determinant and inverse).  m_i non-null: u_i is the grid momentum.
+inline void g2p_update_F(pic_context* ctx, const pic_grid* m_i, const pic_grid& u_i,
+                         const pic_pfield& c_p, double dt, const pic_pfield& F,
+                         const pic_pfield& det, const pic_pfield& Finv, const SplineData& sd)
+{
+    check(pic_apic_g2p_update_F(ctx, sd.order, sd.entity, sd.dtype, &sd.mesh, sd.np, &sd.x, m_i,
+                                &u_i, &c_p, dt, &F, &det, &Finv, sd.bins));
+}
+
 }  // namespace APIC
 
 namespace PolyPIC {

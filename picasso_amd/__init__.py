@@ -211,6 +211,31 @@ class APIC:
                                          _bins_ptr(sd.bins))
         check(rc, "pic_apic_g2p_normalized")
 
+    @staticmethod
+    def g2p_update_F(ctx, u_i, c_p, dt, F, det, Finv, sd, m_i=None):
+        """APIC::g2p fused with the velocity gradient and the FP64
+        deformation-gradient update F <- (I + dt grad u) F, det(F), F^-1.
+        With m_i given, u_i is the grid momentum and the velocity mu/m is
+        formed on the fly.  F [np, 9] (updated in place), det [np, 1] and
+        Finv [np, 9] are float64."""
+        import torch
+        x = sd.x
+        mesh = sd.mesh.c()
+        for t, nm in ((F, "F"), (det, "det"), (Finv, "Finv")):
+            if t.dtype != torch.float64:
+                raise TypeError(f"{nm} must be float64")
+        rc = lib.pic_apic_g2p_update_F(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
+                                       ctypes.byref(mesh), x.shape[0],
+                                       ctypes.byref(_pfield(x, 3, "x")),
+                                       ctypes.byref(m_i.c()) if m_i is not None else None,
+                                       ctypes.byref(u_i.c()),
+                                       ctypes.byref(_pfield(c_p, 12, "c_p")), float(dt),
+                                       ctypes.byref(_pfield(F, 9, "F")),
+                                       ctypes.byref(_pfield(det, 1, "det")),
+                                       ctypes.byref(_pfield(Finv, 9, "Finv")),
+                                       _bins_ptr(sd.bins))
+        check(rc, "pic_apic_g2p_update_F")
+
 
 class PolyPIC:
     @staticmethod

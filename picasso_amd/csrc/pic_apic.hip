@@ -47,8 +47,9 @@ __global__ __launch_bounds__(256) void k_apic_p2g_scatter(ApicArgs<T> a)
 }
 
 // ---------------------------------------------------------------------------
-// g2p, gather (any particle order), same iteration scheme.
-template <int ORDER, bool STAG, typename T>
+// g2p, gather (any particle order), same iteration scheme.  With GF the
+// velocity gradient is formed alongside and applied to F.
+template <int ORDER, bool STAG, typename T, bool GF = false>
 __global__ __launch_bounds__(256) void k_apic_g2p_gather(ApicArgs<T> a)
 {
     const int64_t n = a.list ? int64_t(a.list[0]) : a.np;
@@ -59,13 +60,13 @@ __global__ __launch_bounds__(256) void k_apic_g2p_gather(ApicArgs<T> a)
         const int64_t pb = a.x.base(p);
         T xp[3] = {a.x.at(pb, 0), a.x.at(pb, 1), a.x.at(pb, 2)};
         Spline<ORDER, T> sd;
-        eval_spline<ORDER, T, false, true>(a.lat, xp, sd);
+        eval_spline<ORDER, T, GF, true>(a.lat, xp, sd);
         if (!stencil_inside(sd, a.gmu)) {
             atomicOr(a.err, ERRBIT_BOUNDS);
             continue;
         }
-        T cp[12];
-        apic_g2p_particle<ORDER, STAG>(a, sd, [&](int i, int j, int k, T* ue) {
+        T cp[12], L[9];
+        apic_g2p_particle<ORDER, STAG, GF>(a, sd, [&](int i, int j, int k, T* ue) {
             const int ni = sd.s0[0] + i, nj = sd.s0[1] + j, nk = sd.s0[2] + k;
             const int64_t off = a.gmu.off(ni, nj, nk);
             const int64_t moff = a.gnm.data ? a.gnm.off(ni, nj, nk) : 0;
@@ -75,10 +76,11 @@ __global__ __launch_bounds__(256) void k_apic_g2p_gather(ApicArgs<T> a)
 #pragma unroll
                 for (int d = 0; d < 3; ++d) ue[d] = node_velocity(a.gmu, a.gnm, off, moff, d);
             }
-        }, cp);
+        }, cp, L);
         const int64_t cb = a.c.base(p);
 #pragma unroll
         for (int r = 0; r < 12; ++r) a.c.at(cb, r) = cp[r];
+        if constexpr (GF) store_F_update(a, p, L);
     }
 }
 
@@ -110,6 +112,12 @@ struct G2PGatherLaunch {
     template <int O, bool S, typename U>
     void operator()()
     {
+        if constexpr (!S) {
+            if (a.F.data) {
+                k_apic_g2p_gather<O, S, U, true><<<launch_blocks(a.list, a.np), 256, 0, st>>>(a);
+                return;
+            }
+        }
         k_apic_g2p_gather<O, S, U><<<launch_blocks(a.list, a.np), 256, 0, st>>>(a);
     }
 };
@@ -139,7 +147,8 @@ int apic_p2g_list(pic_context* ctx, int order, int entity, int dtype, const pic_
 
 int apic_g2p_list(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                   int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
-                  const pic_pfield& c, const uint32_t* perm, const uint32_t* list)
+                  const pic_pfield& c, const uint32_t* perm, const uint32_t* list,
+                  const FUpdate* fu)
 {
     const bool stag = stag_dim(entity) >= 0;
     TimedLaunch tl(ctx, "apic_g2p_stale");
@@ -149,6 +158,7 @@ int apic_g2p_list(pic_context* ctx, int order, int entity, int dtype, const pic_
             ctx->stream};
         L.a.list = list;
         if (gmass) L.a.gnm = make_gview<double>(*gmass);
+        set_F_update(L.a, fu);
         dispatch_order_stag<double>(order, stag, L);
     } else {
         G2PGatherLaunch<float> L{
@@ -156,6 +166,7 @@ int apic_g2p_list(pic_context* ctx, int order, int entity, int dtype, const pic_
             ctx->stream};
         L.a.list = list;
         if (gmass) L.a.gnm = make_gview<float>(*gmass);
+        set_F_update(L.a, fu);
         dispatch_order_stag<float>(order, stag, L);
     }
     return launch_status();
@@ -220,7 +231,7 @@ extern "C" int pic_apic_p2g(pic_context* ctx, int order, int entity, int dtype,
 static int apic_g2p_impl(pic_context* ctx, int order, int entity, int dtype,
                          const pic_mesh* mesh, int64_t np, const pic_pfield* x,
                          const pic_grid* u_i, const pic_grid* m_i, const pic_pfield* c_p,
-                         const pic_bins* bins)
+                         const pic_bins* bins, const FUpdate* fu = nullptr)
 {
     if (!ctx || !mesh || !(mesh->h > 0.0) || order < 1 || order > 3 ||
         !valid_entity(entity) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
@@ -228,13 +239,18 @@ static int apic_g2p_impl(pic_context* ctx, int order, int entity, int dtype,
         (m_i && !same_window(*m_i, *u_i)))
         return PIC_ERR_ARG;
     if (np > 0 && (!valid_pfield(x) || !valid_pfield(c_p))) return PIC_ERR_ARG;
+    if (fu && ((entity != PIC_NODE && entity != PIC_CELL) ||
+               (np > 0 && (!valid_pfield(fu->F) || !valid_pfield(fu->det) ||
+                           !valid_pfield(fu->Finv)))))
+        return PIC_ERR_ARG;
     if (bins && !bins->perm) return PIC_ERR_ARG;
     if (np == 0) return PIC_OK;
     DeviceGuard g(ctx->device);
     if (!g.ok) return PIC_ERR_HIP;
     const bool stag = stag_dim(entity) >= 0;
     if (bins) {
-        int rc = apic_g2p_brick(ctx, order, entity, dtype, *mesh, np, *x, *u_i, m_i, *c_p, *bins);
+        int rc = apic_g2p_brick(ctx, order, entity, dtype, *mesh, np, *x, *u_i, m_i, *c_p, *bins,
+                                fu);
         if (rc != PIC_ERR_UNSUPPORTED) return rc;
     }
     TimedLaunch tl(ctx, "apic_g2p");
@@ -244,12 +260,14 @@ static int apic_g2p_impl(pic_context* ctx, int order, int entity, int dtype,
             make_args<double>(ctx, entity, *mesh, np, x, nullptr, c_p, nullptr, *u_i, perm),
             ctx->stream};
         if (m_i) L.a.gnm = make_gview<double>(*m_i);
+        set_F_update(L.a, fu);
         dispatch_order_stag<double>(order, stag, L);
     } else {
         G2PGatherLaunch<float> L{
             make_args<float>(ctx, entity, *mesh, np, x, nullptr, c_p, nullptr, *u_i, perm),
             ctx->stream};
         if (m_i) L.a.gnm = make_gview<float>(*m_i);
+        set_F_update(L.a, fu);
         dispatch_order_stag<float>(order, stag, L);
     }
     return launch_status();
@@ -270,4 +288,15 @@ extern "C" int pic_apic_g2p_normalized(pic_context* ctx, int order, int entity, 
 {
     if (!m_i) return PIC_ERR_ARG;
     return apic_g2p_impl(ctx, order, entity, dtype, mesh, np, x, mu_i, m_i, c_p, bins);
+}
+
+extern "C" int pic_apic_g2p_update_F(pic_context* ctx, int order, int entity, int dtype,
+                                     const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                                     const pic_grid* m_i, const pic_grid* u_i,
+                                     const pic_pfield* c_p, double dt, const pic_pfield* F,
+                                     const pic_pfield* det, const pic_pfield* Finv,
+                                     const pic_bins* bins)
+{
+    const FUpdate fu{F, det, Finv, dt};
+    return apic_g2p_impl(ctx, order, entity, dtype, mesh, np, x, u_i, m_i, c_p, bins, &fu);
 }

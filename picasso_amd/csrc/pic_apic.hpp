@@ -19,7 +19,39 @@ struct ApicArgs {
     GView<T> gnm;          // g2p with fused normalisation: grid mass (else null data)
     int sdim;              // staggered component (entity dim)
     int* err;
+    // g2p with the deformation-gradient update fused in (F.data null: off).
+    PField<double> F, det, Finv;
+    double dt;
 };
+
+// Deformation-gradient outputs of a fused g2p (pic_apic_g2p_update_F).
+struct FUpdate {
+    const pic_pfield* F;
+    const pic_pfield* det;
+    const pic_pfield* Finv;
+    double dt;
+};
+
+// Velocity gradient L of particle p applied to its F: F <- (I + dt L) F,
+// det and F^{-1} stored (FP64).
+template <typename T>
+__device__ __forceinline__ void store_F_update(const ApicArgs<T>& a, int64_t p, const T L[9])
+{
+    const int64_t fb = a.F.base(p), ib = a.Finv.base(p);
+    double Ld[9], Fp[9], inv[9], dd;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        Ld[q] = double(L[q]);
+        Fp[q] = a.F.at(fb, q);
+    }
+    update_F(Ld, a.dt, Fp, dd, inv);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        a.F.at(fb, q) = Fp[q];
+        a.Finv.at(ib, q) = inv[q];
+    }
+    a.det.at(a.det.base(p), 0) = dd;
+}
 
 // Particle state for p2g.  Staggered transfers keep only row `sdim` of
 // u_p and B_p, stored at index 0.
@@ -123,17 +155,21 @@ __device__ __forceinline__ void apic_p2g_particle(const ApicArgs<T>& a,
 
 // g2p of one particle: fetch(i, j, k, ue[ncomp]) reads the grid; the 12
 // components of c_p are returned in cp (row-major 4x3).  A staggered g2p
-// leaves zeros in the other components (Picasso_APIC.hpp:505-543).
-template <int ORDER, bool STAG, typename T, typename Fetch>
+// leaves zeros in the other components (Picasso_APIC.hpp:505-543).  With
+// GRAD (collocated only; the spline must carry gradients) L also receives
+// the velocity gradient L(a, b) = sum_n u_n(a) dw_n/dx_b (G2P::gradient,
+// Picasso_ParticleInterpolation.hpp:165-181) from the same grid values.
+template <int ORDER, bool STAG, bool GRAD = false, typename T, typename Fetch>
 __device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
                                                   const Spline<ORDER, T>& sd, Fetch&& fetch,
-                                                  T cp[12])
+                                                  T cp[12], T* L = nullptr)
 {
     constexpr int NK = Knots<ORDER>::NK;
     constexpr int NC = STAG ? 1 : 3;
+    static_assert(!(GRAD && STAG), "velocity gradient of collocated fields only");
     // v_p = sum w u and B_p = sum w u d^T (Picasso_APIC.hpp:439-471), summed
     // one dimension at a time: the k sums first (weights w_k and w_k d_k),
-    // then j, then i.
+    // then j, then i.  The gradient adds the sums with g_k, g_j and g_i.
     T wd[3][NK];
 #pragma unroll
     for (int e = 0; e < 3; ++e)
@@ -141,21 +177,23 @@ __device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
         for (int n = 0; n < NK; ++n) wd[e][n] = sd.w[e][n] * sd.d[e][n];
     T up[NC];
     T B[NC][3];
+    T G[GRAD ? NC : 1][3];
 #pragma unroll
     for (int d = 0; d < NC; ++d) {
         up[d] = T(0);
         B[d][0] = B[d][1] = B[d][2] = T(0);
+        if constexpr (GRAD) G[d][0] = G[d][1] = G[d][2] = T(0);
     }
 #pragma unroll
     for (int i = 0; i < NK; ++i) {
-        T Si[NC], Sdi[NC], Ti[NC];
+        T Si[NC], Sdi[NC], Ti[NC], Sgi[NC], Ri[NC];
 #pragma unroll
-        for (int d = 0; d < NC; ++d) Si[d] = Sdi[d] = Ti[d] = T(0);
+        for (int d = 0; d < NC; ++d) Si[d] = Sdi[d] = Ti[d] = Sgi[d] = Ri[d] = T(0);
 #pragma unroll
         for (int j = 0; j < NK; ++j) {
-            T sij[NC], tij[NC];
+            T sij[NC], tij[NC], rij[NC];
 #pragma unroll
-            for (int d = 0; d < NC; ++d) sij[d] = tij[d] = T(0);
+            for (int d = 0; d < NC; ++d) sij[d] = tij[d] = rij[d] = T(0);
 #pragma unroll
             for (int k = 0; k < NK; ++k) {
                 T ue[NC];
@@ -164,6 +202,7 @@ __device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
                 for (int d = 0; d < NC; ++d) {
                     sij[d] += sd.w[2][k] * ue[d];
                     tij[d] += wd[2][k] * ue[d];
+                    if constexpr (GRAD) rij[d] += sd.g[2][k] * ue[d];
                 }
             }
 #pragma unroll
@@ -171,6 +210,10 @@ __device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
                 Si[d] += sd.w[1][j] * sij[d];
                 Sdi[d] += wd[1][j] * sij[d];
                 Ti[d] += sd.w[1][j] * tij[d];
+                if constexpr (GRAD) {
+                    Sgi[d] += sd.g[1][j] * sij[d];
+                    Ri[d] += sd.w[1][j] * rij[d];
+                }
             }
         }
 #pragma unroll
@@ -179,6 +222,11 @@ __device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
             B[d][0] += wd[0][i] * Si[d];
             B[d][1] += sd.w[0][i] * Sdi[d];
             B[d][2] += sd.w[0][i] * Ti[d];
+            if constexpr (GRAD) {
+                G[d][0] += sd.g[0][i] * Si[d];
+                G[d][1] += sd.w[0][i] * Sgi[d];
+                G[d][2] += sd.w[0][i] * Ri[d];
+            }
         }
     }
     if constexpr (STAG) {
@@ -195,6 +243,12 @@ __device__ __forceinline__ void apic_g2p_particle(const ApicArgs<T>& a,
             cp[d] = up[d];
 #pragma unroll
             for (int q = 0; q < 3; ++q) cp[(1 + q) * 3 + d] = B[d][q];
+        }
+        if constexpr (GRAD) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) L[d * 3 + q] = G[d][q];
         }
     }
 }
@@ -218,6 +272,8 @@ inline ApicArgs<T> make_args(pic_context* ctx, int entity, const pic_mesh& mesh,
     a.gnm = GView<T>{};
     a.sdim = stag_dim(entity) < 0 ? 0 : stag_dim(entity);
     a.err = ctx->d_err;
+    a.F = a.det = a.Finv = PField<double>{};
+    a.dt = 0.0;
     return a;
 }
 
@@ -248,7 +304,8 @@ int apic_p2g_list(pic_context* ctx, int order, int entity, int dtype, const pic_
                   const uint32_t* list);
 int apic_g2p_list(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                   int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
-                  const pic_pfield& c, const uint32_t* perm, const uint32_t* list);
+                  const pic_pfield& c, const uint32_t* perm, const uint32_t* list,
+                  const FUpdate* fu);
 
 // Brick scheme (pic_apic_brick.hip).  Return PIC_ERR_UNSUPPORTED to fall
 // back to the scatter/gather kernels.
@@ -257,7 +314,18 @@ int apic_p2g_brick(pic_context* ctx, int order, int entity, int dtype, const pic
                    const pic_grid& gm, const pic_grid& gmu, const pic_bins& bins);
 int apic_g2p_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                    int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
-                   const pic_pfield& c, const pic_bins& bins);
+                   const pic_pfield& c, const pic_bins& bins, const FUpdate* fu);
+
+// Attach the F-update outputs to the arguments of a g2p.
+template <typename T>
+inline void set_F_update(ApicArgs<T>& a, const FUpdate* fu)
+{
+    if (!fu) return;
+    a.F = make_pfield<double>(*fu->F);
+    a.det = make_pfield<double>(*fu->det);
+    a.Finv = make_pfield<double>(*fu->Finv);
+    a.dt = fu->dt;
+}
 
 // Grid velocity at a node: the stored value, or momentum / mass (0 where the
 // mass is 0) with the division in FP64 when normalisation is fused in.

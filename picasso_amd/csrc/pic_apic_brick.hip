@@ -363,7 +363,7 @@ __global__ __launch_bounds__((P2GShape<ORDER, T>::NT), (P2GShape<ORDER, T>::WAVE
     walk_bricks<true>(g, slot, [&](uint32_t b) { p2g_one_brick<ORDER, T>(g, b, lds, rng); });
 }
 
-template <int BB, int ORDER, typename T>
+template <int BB, int ORDER, typename T, bool GF>
 __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, uint32_t b, T* us)
 {
     constexpr int FN = BB + ORDER;
@@ -408,35 +408,36 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, uint32_t b,
             const uint32_t s = s0 + uint32_t(r * BT);
             if (s >= end) continue;
             Spline<ORDER, T> sd;
-            eval_spline<ORDER, T, false, true>(a.lat, xp[r], sd);
+            eval_spline<ORDER, T, GF, true>(a.lat, xp[r], sd);
             const int li = sd.s0[0] - f0[0], lj = sd.s0[1] - f0[1], lk = sd.s0[2] - f0[2];
             if (!(li >= 0 && li < BB && lj >= 0 && lj < BB && lk >= 0 && lk < BB)) {
                 push_stale(g, s);
                 continue;
             }
             const int lbase = (li * FN + lj) * FN + lk;
-            T cp[12];
-            apic_g2p_particle<ORDER, false>(a, sd, [&](int i, int j, int k, T* ue) {
+            T cp[12], L[9];
+            apic_g2p_particle<ORDER, false, GF>(a, sd, [&](int i, int j, int k, T* ue) {
                 const int q = lbase + (i * FN + j) * FN + k;
                 ue[0] = us[q];
                 ue[1] = us[FV + q];
                 ue[2] = us[2 * FV + q];
-            }, cp);
+            }, cp, L);
             const int64_t cb = a.c.base(pp[r]);
 #pragma unroll
             for (int e = 0; e < 12; ++e) a.c.at(cb, e) = cp[e];
+            if constexpr (GF) store_F_update(a, pp[r], L);
         }
     }
     __syncthreads();
 }
 
-template <int BB, int ORDER, typename T>
+template <int BB, int ORDER, typename T, bool GF>
 __global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
 {
     constexpr int FN = BB + ORDER;
     __shared__ T us[3 * FN * FN * FN];
     __shared__ uint32_t slot[2];
-    walk_bricks<false>(g, slot, [&](uint32_t b) { g2p_one_brick<BB, ORDER, T>(g, b, us); });
+    walk_bricks<false>(g, slot, [&](uint32_t b) { g2p_one_brick<BB, ORDER, T, GF>(g, b, us); });
 }
 
 // Knot offset of the bins for (order, entity), or false when the bin
@@ -512,11 +513,18 @@ template <typename T>
 struct G2PBrickLaunch {
     pic_context* ctx;
     const BrickArgs<T>& g;
+    template <int BB, int O, bool GF>
+    void launch()
+    {
+        const unsigned nbk =
+            persistent_blocks(ctx, k_apic_g2p_brick<BB, O, T, GF>, BT, 0, g.nbricks);
+        k_apic_g2p_brick<BB, O, T, GF><<<nbk, BT, 0, ctx->stream>>>(g);
+    }
     template <int BB, int O>
     void go()
     {
-        const unsigned nbk = persistent_blocks(ctx, k_apic_g2p_brick<BB, O, T>, BT, 0, g.nbricks);
-        k_apic_g2p_brick<BB, O, T><<<nbk, BT, 0, ctx->stream>>>(g);
+        if (g.a.F.data) launch<BB, O, true>();
+        else launch<BB, O, false>();
     }
     template <int BB>
     void by_order(int order)
@@ -548,12 +556,13 @@ static int p2g_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
 template <typename T>
 static int g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh& mesh, int64_t np,
                          const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
-                         const pic_pfield& c, const pic_bins& bins, int dtype)
+                         const pic_pfield& c, const pic_bins& bins, int dtype, const FUpdate* fu)
 {
     BrickArgs<T> g;
     if (!make_brick_args<T>(ctx, order, entity, mesh, np, x, nullptr, c, nullptr, gu, gmass, bins,
                             g))
         return PIC_ERR_UNSUPPORTED;
+    set_F_update(g.a, fu);
     {
         TimedLaunch tl(ctx, "apic_g2p");
         G2PBrickLaunch<T> L{ctx, g};
@@ -561,7 +570,8 @@ static int g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
         else L.template by_order<8>(order);
         if (launch_status()) return PIC_ERR_HIP;
     }
-    return apic_g2p_list(ctx, order, entity, dtype, mesh, np, x, gu, gmass, c, bins.perm, g.stale);
+    return apic_g2p_list(ctx, order, entity, dtype, mesh, np, x, gu, gmass, c, bins.perm, g.stale,
+                         fu);
 }
 
 int apic_p2g_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
@@ -575,11 +585,11 @@ int apic_p2g_brick(pic_context* ctx, int order, int entity, int dtype, const pic
 
 int apic_g2p_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                    int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
-                   const pic_pfield& c, const pic_bins& bins)
+                   const pic_pfield& c, const pic_bins& bins, const FUpdate* fu)
 {
     if (dtype == PIC_F32)
-        return g2p_brick_run<float>(ctx, order, entity, mesh, np, x, gu, gmass, c, bins, dtype);
-    return g2p_brick_run<double>(ctx, order, entity, mesh, np, x, gu, gmass, c, bins, dtype);
+        return g2p_brick_run<float>(ctx, order, entity, mesh, np, x, gu, gmass, c, bins, dtype, fu);
+    return g2p_brick_run<double>(ctx, order, entity, mesh, np, x, gu, gmass, c, bins, dtype, fu);
 }
 
 }  // namespace pic

@@ -362,4 +362,25 @@ __device__ __forceinline__ void inverse3(const T* a, T det, T* r)
     r[8] = (a[0] * a[4] - a[1] * a[3]) * di;
 }
 
+// F <- (I + dt L) F, then det(F) and F^{-1}, all in FP64: the matrix
+// product (Picasso_BatchedLinearAlgebra.hpp:3806-3834), operator! (:4335-4350)
+// and inverse (:4624-4685).
+__device__ __forceinline__ void update_F(const double L[9], double dt, double F[9], double& det,
+                                         double Finv[9])
+{
+    double A[9], Fn[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) A[q] = (q % 4 == 0 ? 1.0 : 0.0) + dt * L[q];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            Fn[i * 3 + j] = A[i * 3 + 0] * F[0 * 3 + j] + A[i * 3 + 1] * F[1 * 3 + j] +
+                            A[i * 3 + 2] * F[2 * 3 + j];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) F[q] = Fn[q];
+    det = det3(F);
+    inverse3(F, det, Finv);
+}
+
 }  // namespace pic

@@ -120,25 +120,17 @@ __global__ __launch_bounds__(256) void k_update_F(int64_t np, PField<TL> L, doub
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= np) return;
     const int64_t lb = L.base(p), fb = F.base(p);
-    double A[9], Fp[9], Fn[9];
+    double Lp[9], Fp[9], inv[9], dd;
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-        A[q] = (q % 4 == 0 ? 1.0 : 0.0) + dt * (double)L.at(lb, q);
+        Lp[q] = (double)L.at(lb, q);
         Fp[q] = F.at(fb, q);
     }
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j)
-            Fn[i * 3 + j] = A[i * 3 + 0] * Fp[0 * 3 + j] + A[i * 3 + 1] * Fp[1 * 3 + j] +
-                            A[i * 3 + 2] * Fp[2 * 3 + j];
-    const double dd = det3(Fn);
-    double inv[9];
-    inverse3(Fn, dd, inv);
+    update_F(Lp, dt, Fp, dd, inv);
     const int64_t ib = Finv.base(p);
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
-        F.at(fb, q) = Fn[q];
+        F.at(fb, q) = Fp[q];
         Finv.at(ib, q) = inv[q];
     }
     det.at(det.base(p), 0) = dd;

@@ -460,3 +460,62 @@ def test_active_brick_list(pa, wl0):
     act = gu.host(res.active)
     assert act[0] == len(want)
     np.testing.assert_array_equal(act[1:1 + len(want)], want)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("binned", [False, True])
+@pytest.mark.parametrize("fused_norm", [False, True])
+def test_g2p_update_F_matches_oracle(pa, wl0, order, dtype, binned, fused_norm):
+    """pic_apic_g2p_update_F against the oracle's g2p, velocity gradient and
+    FP64 F update run one after the other (FP64 F, det, F^-1 at 1e-12 for
+    FP64 inputs; FP32 inputs within the FP32 bar)."""
+    import gpu_util as gu
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    f32 = dtype == "f32"
+    xh = wl0.x.astype(np.float32).astype(np.float64) if f32 else wl0.x
+    lo, n = wl0.node_window
+    mesh = pa.Mesh(wl0.low, wl0.h)
+    x = gu.soa(xh, tdt)
+    rng = np.random.default_rng(70 + order)
+    P = len(xh)
+    F0 = np.eye(3) + rng.normal(0, 0.01, size=(P, 3, 3))
+    dt = 1e-3
+    if fused_norm:
+        gmh = rng.uniform(0.5, 1.5, size=tuple(n))
+        gmh[rng.random(gmh.shape) < 0.05] = 0.0
+        gmuh = rng.normal(size=tuple(n) + (3,))
+        if f32:
+            gmh = gmh.astype(np.float32).astype(np.float64)
+            gmuh = gmuh.astype(np.float32).astype(np.float64)
+        uh = orpic.grid_normalize(gmh, gmuh)
+        m_i = pa.GridField(gu.dev(gmh, tdt))
+        u_i = pa.GridField(gu.dev(gmuh, tdt))
+    else:
+        uh = rng.normal(size=tuple(n) + (3,))
+        if f32:
+            uh = uh.astype(np.float32).astype(np.float64)
+        m_i = None
+        u_i = pa.GridField(gu.dev(uh, tdt))
+    bins = None
+    if binned:
+        bins = pa.bin_sort(gu.ctx(), mesh, x, _binning(pa, wl0, order, 0, (4, 4, 4)))
+    sd = pa.create_spline(0, order, mesh, x, bins)
+    cg = gu.soa(np.zeros((P, 12)), tdt)
+    F = gu.soa(F0.reshape(P, 9))
+    det = gu.soa(np.zeros((P, 1)))
+    Finv = gu.soa(np.zeros((P, 9)))
+    pa.APIC.g2p_update_F(gu.ctx(), u_i, cg, dt, F, det, Finv, sd, m_i=m_i)
+    gu.ctx().synchronize()
+    rc = orpic.apic_g2p(order, 0, wl0.low, wl0.h, lo, n, xh, uh)
+    rg = orpic.g2p_gradu(order, 0, wl0.low, wl0.h, lo, n, xh, uh)
+    rF, rdet, rFinv = orpic.update_F(rg, dt, F0)
+    tol = F32_TOL if f32 else F64_TOL
+    assert gu.rel_inf(gu.host(cg).reshape(-1, 4, 3)[:, 0], rc[:, 0]) < tol
+    assert gu.rel_inf(gu.host(cg).reshape(-1, 4, 3)[:, 1:], rc[:, 1:]) < tol * 10
+    # F, det, F^-1 are FP64; with FP32 grid data the gradient carries FP32
+    # rounding, scaled by dt in F.
+    ftol = 1e-12 if not f32 else 1e-6
+    assert gu.rel_inf(gu.host(F).reshape(-1, 3, 3), rF) < ftol
+    assert gu.rel_inf(gu.host(det)[:, 0], rdet) < ftol
+    assert gu.rel_inf(gu.host(Finv).reshape(-1, 3, 3), rFinv) < ftol
