@@ -77,6 +77,7 @@ class Step:
         """Store the particles in bin order (what a PIC code does when it
         re-sorts its particle arrays); later sorts still run in full."""
         self.pa.bin_sort(self.ctx, self.mesh, self.x, self.geom, out=self.bins)
+        self.ctx.synchronize()  # raises if a particle fell outside the bins (perm undefined then)
         perm = self.bins.perm.long()
         for name in ("x", "m", "c", "c0"):
             t = getattr(self, name)
@@ -140,6 +141,140 @@ def cpu_baseline(wl, reps=3):
                       f"median of {reps} after 1 warm-up, {t * 1e3:.1f} ms/step"}
 
 
+def _time(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def _kernel_avgs(ctx, families):
+    out = {}
+    for f in families:
+        ms, n = ctx.kernel_time(f)
+        if n:
+            out[f] = ms / n
+    return out
+
+
+def config_plumbing(pa, ctx, synth):
+    """32^3 cells x 8 ppc, FP64: sort -> p2g -> g2p (normalisation fused)."""
+    import torch
+    wl = synth.apic_uniform(32, 8)
+    st = Step(pa, ctx, wl, torch.float64, wl.x, wl.m, wl.c)
+    st.reorder()
+    ms = _time(st.run, 20, 3)
+    return {"workload": "APIC uniform 32^3 cells x 8 ppc, quadratic node spline, FP64",
+            "particles": wl.np, "ms_per_step": ms, "particles_per_s": wl.np / (ms * 1e-3)}
+
+
+def config_f_update(pa, ctx, synth, steps=10):
+    """Cube [1/4, 3/4]^3 of a 256^3 mesh at 8 ppc, FP32 particles and grid,
+    FP64 F: sort -> p2g -> g2p fused with grad u and F <- (I + dt grad u) F,
+    det F, F^-1 (FP64), 10 consecutive steps."""
+    import torch
+    wl = synth.apic_cube(256, 8)
+    P = wl.np
+    st = Step(pa, ctx, wl, torch.float32, wl.x.astype(np.float32), wl.m.astype(np.float32),
+              wl.c.astype(np.float32))
+    st.reorder()
+    perm = st.bins.perm.long().cpu().numpy()
+    F = torch.as_tensor(np.ascontiguousarray(wl.F.reshape(P, 9)[perm].T), device="cuda:0").t()
+    F0 = F.clone()
+    det = torch.empty((1, P), dtype=torch.float64, device="cuda:0").t()
+    Finv = torch.empty((9, P), dtype=torch.float64, device="cuda:0").t()
+    dt = 1e-3
+
+    def step():
+        pa.bin_sort(ctx, st.mesh, st.x, st.geom, out=st.bins)
+        sd = pa.create_spline(pa.FieldLocation.Node, 2, st.mesh, st.x, st.bins)
+        pa.APIC.p2g(ctx, st.m, st.c, st.gm, st.gmu, sd, zero_first=True)
+        pa.APIC.g2p_update_F(ctx, st.gmu, st.c, dt, F, det, Finv, sd, m_i=st.gm)
+
+    step()
+    F.copy_(F0)
+    st.restore()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    ms = _time(step, steps, 0)
+    ctx.set_timing(False)
+    ctx.synchronize()
+    kt = _kernel_avgs(ctx, ["bin_sort", "zero_grid", "apic_p2g", "apic_g2p"])
+    ctx.reset_timing()
+    finite = bool(torch.isfinite(F).all()) and bool(torch.isfinite(Finv).all())
+    return {"workload": "APIC cube [0.25,0.75]^3 of 256^3 cells x 8 ppc, FP32 with FP64 F update, "
+                        f"{steps} consecutive steps",
+            "particles": P, "ms_per_step": ms, "particles_per_s": P / (ms * 1e-3),
+            "kernels_ms": kt, "F_finite": finite}
+
+
+def config_polypic(pa, ctx, synth, steps=5):
+    """128^3 cells x 8 ppc, cubic spline, 27 velocity modes per component,
+    FP32: PolyPIC p2g -> normalise -> g2p (per-particle kernels)."""
+    import torch
+    n, ppc = 128, 8
+    st = synth.Stream(0x5EED0002)
+    h = 1.0 / n
+    ii = np.arange(n)
+    cells = np.stack(np.meshgrid(ii, ii, ii, indexing="ij"), -1).reshape(-1, 3)
+    x = synth._fill_cells(st, cells, ppc, h)
+    P = x.shape[0]
+    low = np.full(3, -synth.HALO * h)
+    dev = "cuda:0"
+    g = torch.Generator(device=dev)
+    g.manual_seed(0x5EED0002)
+    nm = pa.PolyPIC.nmode(3)
+    # modes: N(0, 0.1), mode 0 ~ N(0, 1); SoA [27*3, P] viewed as [P, 81]
+    modes = (torch.randn((nm * 3, P), generator=g, device=dev) * 0.1)
+    modes[:3] = torch.randn((3, P), generator=g, device=dev)
+    modes = modes.t()
+    xt = torch.as_tensor(np.ascontiguousarray(x.T), dtype=torch.float32, device=dev).t()
+    mt = torch.full((1, P), h ** 3 / ppc, dtype=torch.float32, device=dev).t()
+    nn = n + 1 + 2 * synth.HALO
+    gm = pa.GridField(torch.zeros((nn, nn, nn), dtype=torch.float32, device=dev))
+    gmu = pa.GridField(torch.zeros((3, nn, nn, nn), dtype=torch.float32, device=dev).permute(1, 2, 3, 0))
+    gu = pa.GridField(torch.zeros((3, nn, nn, nn), dtype=torch.float32, device=dev).permute(1, 2, 3, 0))
+    cg = torch.empty_like(modes)
+    mesh = pa.Mesh(low, h)
+    # Mesh-cell bins (the cubic knot base), 4^3-bin bricks; particles stored
+    # in bin order once, the sort redone every step.
+    # n + 1 bins per dimension: a position in the last cell may round to
+    # exactly 1.0 in FP32, which lies in the next cell.
+    geom = pa.Binning((synth.HALO,) * 3, (n + 1,) * 3, (4, 4, 4), 0)
+    bins = pa.bin_sort(ctx, mesh, xt, geom)
+    ctx.synchronize()  # raises if a particle fell outside the bins (perm undefined then)
+    perm = bins.perm.long()
+    xt = xt.t()[:, perm].contiguous().t()
+    modes = modes.t()[:, perm].contiguous().t()
+
+    def step():
+        pa.bin_sort(ctx, mesh, xt, geom, out=bins)
+        sd = pa.create_spline(pa.FieldLocation.Node, 3, mesh, xt, bins)
+        pa.PolyPIC.p2g(ctx, mt, modes, modes, gmu, gm, 1e-3, sd, zero_first=True)
+        pa.grid_normalize(ctx, gm, gmu, gu, rel_mass_eps=1e-6)
+        pa.PolyPIC.g2p(ctx, gu, cg, sd)
+
+    step()
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    ms = _time(step, steps, 0)
+    ctx.set_timing(False)
+    ctx.synchronize()
+    kt = _kernel_avgs(ctx, ["bin_sort", "zero_grid", "polypic_p2g", "polypic_p2g_stale", "normalize",
+                            "max_mass", "polypic_g2p"])
+    ctx.reset_timing()
+    return {"workload": "PolyPIC uniform 128^3 cells x 8 ppc, cubic node spline, 27 modes x 3, "
+                        "FP32, cell-bin sort every step, brick p2g, per-particle g2p",
+            "particles": P, "ms_per_step": ms, "particles_per_s": P / (ms * 1e-3),
+            "kernels_ms": kt}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +285,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-shuffled", action="store_true",
                     help="skip the reference step on shuffled storage (profiling runs)")
+    ap.add_argument("--no-other-configs", action="store_true",
+                    help="time only the headline workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the transfer path is built and timed on a single "
@@ -251,6 +388,23 @@ def main():
                 "unit": "GB/s", "frac": None, "traffic": None,
                 "note": "dominant kernel has no algorithmic-byte definition (sort overhead)"}
 
+    others = None
+    if not args.no_other_configs:
+        del s32
+        torch.cuda.empty_cache()
+        others = {}
+        log("config: plumbing 32^3 FP64 ...")
+        others["apic_uniform_32"] = config_plumbing(pa, ctx, synth)
+        log(f"  {others['apic_uniform_32']}")
+        log("config: cube 256^3 FP32 + FP64 F update ...")
+        others["apic_cube_f_update"] = config_f_update(pa, ctx, synth)
+        log(f"  {others['apic_cube_f_update']}")
+        log("config: PolyPIC cubic 27 modes 128^3 FP32 ...")
+        others["polypic_cubic27"] = config_polypic(pa, ctx, synth)
+        log(f"  {others['polypic_cubic27']}")
+        torch.cuda.empty_cache()
+        others["batched_instances"] = None  # batched multi-instance launch: not built
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -286,6 +440,7 @@ def main():
         "step_alg_GBs": step_gbs,
         "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
         "kernels": kt32,
+        "other_configs": others,
     }
     print(json.dumps(out))
 

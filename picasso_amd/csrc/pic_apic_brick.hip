@@ -24,6 +24,7 @@
 // since the sort) is appended to a list that the scatter/gather kernels
 // process afterwards, so results never depend on the bins being current.
 #include "pic_apic.hpp"
+#include "pic_brick.hpp"
 #include "pic_sort.hpp"
 
 namespace pic {
@@ -34,77 +35,8 @@ constexpr int PB = 4;    // p2g brick edge in bins
 template <typename T>
 struct BrickArgs {
     ApicArgs<T> a;
-    const uint32_t* keys;
-    const uint32_t* bin_start;
-    const uint32_t* active;  // [count, ids...] of non-empty bricks, or null
-    uint32_t* walk;          // brick-walk counter, zero at launch
-    uint32_t* stale;         // [count, slots...] of particles for the slow path
-    uint32_t nbricks;
-    int32_t cell_lo[3];
-    int32_t nb[3];
-    int32_t knot_off;  // first knot index = bin index + knot_off
+    PIC_BRICK_WALK_FIELDS
 };
-
-template <int BB, typename T>
-__device__ __forceinline__ void brick_origin(const BrickArgs<T>& g, uint32_t b, int f0[3])
-{
-    const int bk = int(b % uint32_t(g.nb[2]));
-    const int bj = int((b / uint32_t(g.nb[2])) % uint32_t(g.nb[1]));
-    const int bi = int(b / (uint32_t(g.nb[2]) * uint32_t(g.nb[1])));
-    f0[0] = g.cell_lo[0] + bi * BB + g.knot_off;
-    f0[1] = g.cell_lo[1] + bj * BB + g.knot_off;
-    f0[2] = g.cell_lo[2] + bk * BB + g.knot_off;
-}
-
-// Walk indices run over the active list when there is one, else over every
-// brick (empty ones are skipped).
-template <typename T>
-__device__ __forceinline__ uint32_t walk_count(const BrickArgs<T>& g)
-{
-    return g.active ? g.active[0] : g.nbricks;
-}
-
-template <typename T>
-__device__ __forceinline__ uint32_t walk_brick(const BrickArgs<T>& g, uint32_t t)
-{
-    return g.active ? g.active[1 + t] : t;
-}
-
-// Drive body(brick) over the walk.  A workgroup's first index is its
-// blockIdx.  With DYN, each later one is requested from the counter when the
-// previous brick starts (the request's latency hides behind that brick), so
-// bricks start in list order whatever each one costs; every workgroup ends
-// after drawing an index past the end.  `slot` is a two-word LDS buffer
-// (alternating, so no extra barrier is needed when a brick is skipped).
-// Without DYN the walk is a plain grid stride, which suits kernels whose
-// bricks cost about the same (a contended counter then only adds latency).
-template <bool DYN, typename T, typename Body>
-__device__ __forceinline__ void walk_bricks(const BrickArgs<T>& g, uint32_t* slot, Body&& body)
-{
-    const uint32_t nw = walk_count(g);
-    uint32_t t = blockIdx.x;
-    for (int it = 0; t < nw; ++it) {
-        uint32_t nxt = t + gridDim.x;
-        if (DYN && threadIdx.x == 0) nxt = gridDim.x + atomicAdd(g.walk, 1u);
-        const uint32_t b = walk_brick(g, t);
-        if (g.bin_start[b] != g.bin_start[b + 1]) body(b);
-        if constexpr (DYN) {
-            if (threadIdx.x == 0) slot[it & 1] = nxt;
-            __syncthreads();
-            t = slot[it & 1];
-        } else {
-            __syncthreads();
-            t = nxt;
-        }
-    }
-}
-
-template <typename T>
-__device__ __forceinline__ void push_stale(const BrickArgs<T>& g, uint32_t s)
-{
-    const uint32_t at = atomicAdd(g.stale, 1u);
-    g.stale[1 + at] = s;
-}
 
 // Work split of the p2g brick kernel: four work-items per bin of a 4^3
 // brick, one per grid component (mass, momentum x/y/z), each walking every
@@ -124,7 +56,7 @@ __device__ __forceinline__ void push_stale(const BrickArgs<T>& g, uint32_t s)
 //  - bin sums, after the last chunk: [stencil offset][bin][component], the
 //    CPR components of a round contiguous, so a footprint node reads each
 //    (offset, bin) slot with one wide LDS load; staged in rounds of CPR
-//    components to bound the buffer.
+//    components to bound the buffer (34 KB FP32, 17 KB FP64).
 template <int ORDER, typename T>
 struct P2GShape {
     static constexpr int NK = ORDER + 1;
@@ -134,32 +66,22 @@ struct P2GShape {
     static constexpr int BINS = PB * PB * PB;
     static constexpr int NT = BINS * 4;  // work-items per workgroup
     static constexpr int SLOTS = NN * BINS;  // (offset, bin) slots
-    static constexpr int SUM_BUDGET = 34 * 1024;
+    static constexpr int SUM_BUDGET = sizeof(T) == 4 ? 34 * 1024 : 17 * 1024;
     static constexpr int NROUND = 4 * SLOTS * int(sizeof(T)) <= SUM_BUDGET   ? 1
                                   : 2 * SLOTS * int(sizeof(T)) <= SUM_BUDGET ? 2
                                                                               : 4;
     static constexpr int CPR = 4 / NROUND;  // components per round
-    static constexpr int CAP = sizeof(T) == 4 ? 624 : 400;  // particles per chunk
+    // Particles per staging chunk: FP32 fits a typical brick (8 ppc x 64
+    // bins) in one chunk at four workgroups per compute unit; FP64 is
+    // register-limited to three and runs faster with smaller chunks and more
+    // resident workgroups.
+    static constexpr int CAP = sizeof(T) == 4 ? 624 : 200;
     static constexpr int STAGE = CAP * 16;                   // scalars
     static constexpr int SUMS = CPR * SLOTS;
     static constexpr int LDS = STAGE > SUMS ? STAGE : SUMS;
     // Waves per SIMD the register allocation is held to.
     static constexpr int WAVES = sizeof(T) == 8 && ORDER == 2 ? 3 : 1;
 };
-
-constexpr double STALE_F = 4.0;  // staged f0 of a particle the brick skips
-
-// The components of one bin-sum slot, read with one wide LDS load.
-template <typename T, int N>
-struct alignas(sizeof(T) * N) SumSlot {
-    T c[N];
-};
-
-// Scalar offset of group g of staged particle q.
-__device__ __forceinline__ int stage_at(int q, int g)
-{
-    return q * 16 + 4 * (g ^ ((q >> 2) & 3));
-}
 
 // Accumulate one particle's component into the bin's stencil sums.  v and
 // B0..B2 are row `comp - 1` of u_p and B_p for a momentum component; the
@@ -220,23 +142,15 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
 {
     using S = P2GShape<ORDER, T>;
     using T4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
-    constexpr int NK = S::NK, NN = S::NN, FN = S::FN, FV = S::FV, BINS = S::BINS;
+    constexpr int NK = S::NK, NN = S::NN, BINS = S::BINS;
     constexpr int NT = S::NT, CAP = S::CAP;
     const ApicArgs<T>& a = g.a;
     const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
     int f0[3];
     brick_origin<PB>(g, b, f0);
 
-    // Bin ranges [rng[l], rng[BINS + l]) from the sorted keys (empty: 0, 0).
-    for (int l = threadIdx.x; l < 2 * BINS; l += NT) rng[l] = 0u;
-    __syncthreads();
+    bin_ranges<BINS, NT>(g, b, beg, end, rng);
     const uint32_t kb = b * BINS;
-    for (uint32_t s = beg + threadIdx.x; s < end; s += NT) {
-        const uint32_t key = g.keys[s];
-        if (s == beg || g.keys[s - 1] != key) rng[key - kb] = s;
-        if (s + 1 == end || g.keys[s + 1] != key) rng[BINS + key - kb] = s + 1;
-    }
-    __syncthreads();
 
     const int l = int(threadIdx.x) >> 2, comp = int(threadIdx.x) & 3;
     const int grp = comp > 0 ? comp : 1;
@@ -309,45 +223,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
             for (int n = 0; n < NN; ++n) dst[n * BINS * S::CPR] = acc[n];
         }
         __syncthreads();
-        for (int q = threadIdx.x; q < FV; q += NT) {
-            const int fi = q / (FN * FN), fj = (q / FN) % FN, fk = q % FN;
-            double tot[S::CPR];
-#pragma unroll
-            for (int cc = 0; cc < S::CPR; ++cc) tot[cc] = 0.0;
-#pragma unroll 1
-            for (int oi = 0; oi < NK; ++oi)
-#pragma unroll
-                for (int oj = 0; oj < NK; ++oj)
-#pragma unroll
-                    for (int ok = 0; ok < NK; ++ok) {
-                        const int bi = fi - oi, bj = fj - oj, bk = fk - ok;
-                        if (bi < 0 || bi >= PB || bj < 0 || bj >= PB || bk < 0 || bk >= PB)
-                            continue;
-                        const int slot = ((oi * NK + oj) * NK + ok) * BINS + (bi * PB + bj) * PB + bk;
-                        const SumSlot<T, S::CPR> v =
-                            *reinterpret_cast<const SumSlot<T, S::CPR>*>(lds + slot * S::CPR);
-#pragma unroll
-                        for (int cc = 0; cc < S::CPR; ++cc) tot[cc] += double(v.c[cc]);
-                    }
-            bool any = false;
-#pragma unroll
-            for (int cc = 0; cc < S::CPR; ++cc) any = any || tot[cc] != 0.0;
-            if (!any) continue;
-            const int ni = f0[0] + fi, nj = f0[1] + fj, nk = f0[2] + fk;
-            if (!(a.gmu.covers(0, ni, ni) && a.gmu.covers(1, nj, nj) && a.gmu.covers(2, nk, nk))) {
-                atomicOr(a.err, ERRBIT_BOUNDS);
-                continue;
-            }
-            const int64_t off = a.gmu.off(ni, nj, nk);
-#pragma unroll
-            for (int cc = 0; cc < S::CPR; ++cc) {
-                const int c = r_lo + cc;
-                const T v = T(tot[cc]);
-                if (v == T(0)) continue;
-                if (c == 0) atomic_add(a.gm.data + a.gm.off(ni, nj, nk), v);
-                else atomic_add(a.gmu.data + off + (c - 1) * a.gmu.cs, v);
-            }
-        }
+        gather_flush<NK, PB, NT, S::CPR>(lds, f0, r_lo, 1, a.gm, a.gmu, a.err);
         __syncthreads();
     }
 }
@@ -440,54 +316,16 @@ __global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
     walk_bricks<false>(g, slot, [&](uint32_t b) { g2p_one_brick<BB, ORDER, T, GF>(g, b, us); });
 }
 
-// Knot offset of the bins for (order, entity), or false when the bin
-// lattice is not the spline's knot-base lattice.  In half cells: the base is
-// floor(t_node - centred + order2) and the bin floor(t_node + dual).
-static bool aligned_knot_offset(int order, int entity, int dual, int* knot_off)
-{
-    if (entity != PIC_NODE && entity != PIC_CELL) return false;
-    const int halves = (order == 2 ? 1 : 0) - (entity == PIC_CELL ? 1 : 0) - (dual ? 1 : 0);
-    if (halves != 0 && halves != -2) return false;
-    *knot_off = halves / 2 + (order == 1 ? 0 : -1);
-    return true;
-}
-
-// Cubic brick edge of the binning, or 0 when the brick kernels do not
-// support it.
-static int brick_edge(const pic_binning& geom)
-{
-    const int e = geom.brick[0];
-    if (geom.brick[1] != e || geom.brick[2] != e) return 0;
-    return (e == 4 || e == 8) ? e : 0;
-}
-
 template <typename T>
 static bool make_brick_args(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
                             int64_t np, const pic_pfield& x, const pic_pfield* m,
                             const pic_pfield& c, const pic_grid* gm, const pic_grid& gmu,
                             const pic_grid* gmass, const pic_bins& bins, BrickArgs<T>& g)
 {
-    const int bb = brick_edge(bins.geom);
-    if (!bb || !bins.keys || !bins.bin_start || !bins.perm) return false;
-    if (!aligned_knot_offset(order, entity, bins.geom.dual, &g.knot_off)) return false;
-    const int64_t nb = pic_bin_count(&bins.geom);
-    if (nb <= 0) return false;
+    if (!setup_walk(ctx, order, entity, np, bins, g)) return false;
     g.a = make_args<T>(ctx, entity, mesh, np, &x, m, &c, gm, gmu, bins.perm);
     if (gmass) g.a.gnm = make_gview<T>(*gmass);
-    g.keys = bins.keys;
-    g.bin_start = bins.bin_start;
-    g.active = bins.active;
-    g.nbricks = uint32_t(nb);
-    for (int d = 0; d < 3; ++d) {
-        g.cell_lo[d] = bins.geom.cell_lo[d];
-        g.nb[d] = (bins.geom.ncell[d] + bb - 1) / bb;
-    }
-    // scratch: walk counter | stale count | stale slots[np]
-    uint32_t* w = static_cast<uint32_t*>(scratch(ctx, size_t(np + 2) * sizeof(uint32_t)));
-    if (!w) return false;
-    g.walk = w;
-    g.stale = w + 1;
-    return fill_u32(ctx, w, 2, 0u) == PIC_OK;
+    return true;
 }
 
 template <typename T>

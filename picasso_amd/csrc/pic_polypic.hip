@@ -7,6 +7,7 @@
 // lexicographic order of their exponents.  g2p mode coefficients are the
 // Taylor coefficients of the grid interpolant at the particle: per dimension
 // the weight, its gradient, or half its second derivative.
+#include "pic_brick.hpp"
 #include "pic_common.hpp"
 #include "pic_internal.hpp"
 
@@ -28,6 +29,7 @@ struct PolyArgs {
     Lattice lat;
     int64_t np;
     const uint32_t* perm;
+    const uint32_t* list;  // [count, slots...] to process instead of 0..np-1, or null
     PField<T> x, m, u, c;  // positions, mass, velocity modes (NM x 3), field modes
     GView<T> gmu, gm;
     int pcols;             // columns of the particle field (ncomp)
@@ -45,13 +47,26 @@ __device__ __forceinline__ T ypow(T y, int e)
 // PolyPIC::p2g (Picasso_PolyPIC.hpp:39-130, 138-229).  NC is the number of
 // grid components (1 for staggered).
 template <int ORDER, int NC, bool STAG, typename T>
+__device__ __forceinline__ void polypic_p2g_particle(const PolyArgs<T>& a, int64_t p);
+
+// Grid-stride over particles, or over a device-side list of sorted slots
+// (list[0] = count) left by the brick kernel.
+template <int ORDER, int NC, bool STAG, typename T>
 __global__ __launch_bounds__(256) void k_polypic_p2g(PolyArgs<T> a)
+{
+    const int64_t n = a.list ? int64_t(a.list[0]) : a.np;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = a.list ? int64_t(a.list[1 + i]) : i;
+        polypic_p2g_particle<ORDER, NC, STAG, T>(a, a.perm ? int64_t(a.perm[s]) : s);
+    }
+}
+
+template <int ORDER, int NC, bool STAG, typename T>
+__device__ __forceinline__ void polypic_p2g_particle(const PolyArgs<T>& a, int64_t p)
 {
     constexpr int NK = Knots<ORDER>::NK;
     constexpr int NM = Modes<ORDER>::NM;
-    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= a.np) return;
-    const int64_t p = a.perm ? int64_t(a.perm[s]) : s;
 
     const int64_t xb = a.x.base(p);
     T xp[3] = {a.x.at(xb, 0), a.x.at(xb, 1), a.x.at(xb, 2)};
@@ -175,6 +190,233 @@ __global__ __launch_bounds__(256) void k_polypic_g2p(PolyArgs<T> a)
     }
 }
 
+// ---------------------------------------------------------------------------
+// Brick p2g (collocated, three grid components) for particles binned by
+// pic_bin_sort on the spline's knot-base lattice: the APIC brick scheme
+// with the PolyPIC node value.  Four work-items per bin of a 4^3 brick, one
+// per grid component (mass, momentum x/y/z), each walking every particle of
+// its bin and accumulating its component at the NK^3 stencil nodes in
+// registers; the bin sums go through LDS to one atomic per footprint node
+// (gather_flush).  Particles are staged per chunk with coalesced loads:
+// (f0, f1, f2, m) and the rows of A^-1 (A = I + dt U, Picasso_PolyPIC.hpp:
+// 81-87), computed once per particle.  A work-item reads its component's
+// modes from memory and evaluates sum_r c(r, d) b_r(y) at y = A^-1 d_n as a
+// nested Horner polynomial; y is separable in the node offsets,
+// y = P_i + Q_j + R_k.
+template <typename T>
+struct PolyBrickArgs {
+    PolyArgs<T> a;
+    PIC_BRICK_WALK_FIELDS
+};
+
+template <int ORDER, typename T>
+struct PolyBrickShape {
+    static constexpr int PB = 4;
+    static constexpr int NK = ORDER + 1;
+    static constexpr int NN = NK * NK * NK;
+    static constexpr int BINS = PB * PB * PB;
+    static constexpr int NT = BINS * 4;
+    static constexpr int SLOTS = NN * BINS;
+    static constexpr int SUM_BUDGET = sizeof(T) == 4 ? 34 * 1024 : 17 * 1024;
+    static constexpr int NROUND = 4 * SLOTS * int(sizeof(T)) <= SUM_BUDGET   ? 1
+                                  : 2 * SLOTS * int(sizeof(T)) <= SUM_BUDGET ? 2
+                                                                              : 4;
+    static constexpr int CPR = 4 / NROUND;
+    static constexpr int CAP = sizeof(T) == 4 ? 576 : 200;
+    static constexpr int STAGE = CAP * 16;
+    static constexpr int SUMS = CPR * SLOTS;
+    static constexpr int LDS = STAGE > SUMS ? STAGE : SUMS;
+};
+
+// sum over exponents e of C[e0][e1][e2] y0^e0 y1^e1 y2^e2 (E exponents per
+// dimension), innermost dimension first.
+template <int E, typename T>
+__device__ __forceinline__ T nested_horner(const T (&C)[E][E][E], const T y[3])
+{
+    T mid[E];
+#pragma unroll
+    for (int e0 = 0; e0 < E; ++e0) {
+        T inner[E];
+#pragma unroll
+        for (int e1 = 0; e1 < E; ++e1) {
+            T v = C[e0][e1][E - 1];
+#pragma unroll
+            for (int e2 = E - 2; e2 >= 0; --e2) v = v * y[2] + C[e0][e1][e2];
+            inner[e1] = v;
+        }
+        T v = inner[E - 1];
+#pragma unroll
+        for (int e1 = E - 2; e1 >= 0; --e1) v = v * y[1] + inner[e1];
+        mid[e0] = v;
+    }
+    T v = mid[E - 1];
+#pragma unroll
+    for (int e0 = E - 2; e0 >= 0; --e0) v = v * y[0] + mid[e0];
+    return v;
+}
+
+template <int ORDER, typename T>
+__device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g, uint32_t b, T* lds,
+                                                      uint32_t* pidx, uint32_t* rng)
+{
+    using S = PolyBrickShape<ORDER, T>;
+    using T4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
+    constexpr int PB = S::PB, NK = S::NK, NN = S::NN, BINS = S::BINS, NT = S::NT, CAP = S::CAP;
+    constexpr int NM = Modes<ORDER>::NM;
+    constexpr int E = ORDER == 1 ? 2 : 3;
+    constexpr int FIRST = Knots<ORDER>::FIRST;
+    const PolyArgs<T>& a = g.a;
+    const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
+    int f0[3];
+    brick_origin<PB>(g, b, f0);
+    bin_ranges<BINS, NT>(g, b, beg, end, rng);
+    const uint32_t kb = b * BINS;
+
+    const int l = int(threadIdx.x) >> 2, comp = int(threadIdx.x) & 3;
+    const int d = comp > 0 ? comp - 1 : 0;
+    const T dt = (T)a.dt;
+    T acc[NN];
+#pragma unroll
+    for (int n = 0; n < NN; ++n) acc[n] = T(0);
+    const uint32_t b0 = rng[l], b1 = rng[BINS + l];
+    for (uint32_t c0 = beg; c0 < end; c0 += CAP) {
+        const uint32_t c1 = min(end, c0 + uint32_t(CAP));
+        for (uint32_t s = c0 + threadIdx.x; s < c1; s += NT) {
+            const int64_t p = int64_t(a.perm[s]);
+            const int q = int(s - c0);
+            const int lb = int(g.keys[s] - kb);
+            const int eb[3] = {f0[0] + lb / (PB * PB), f0[1] + (lb / PB) % PB, f0[2] + lb % PB};
+            const int64_t xb = a.x.base(p);
+            T4 v;
+            T* fr = &v.x;
+            bool ok = true;
+#pragma unroll
+            for (int dim = 0; dim < 3; ++dim) {
+                double fd;
+                const double kb0 = knot_base<ORDER>(double(a.x.at(xb, dim)), a.lat.o[dim],
+                                                     a.lat.rdx, fd);
+                ok = ok && int(kb0) + FIRST == eb[dim];
+                fr[dim] = T(fd);
+            }
+            if (!ok) {
+                push_stale(g, s);
+                v.x = T(STALE_F);
+            }
+            v.w = a.m.at(a.m.base(p), 0);
+            *reinterpret_cast<T4*>(lds + stage_at(q, 0)) = v;
+            // A = I + dt U, U(r, d) = u_p(1 + r, d) (:81-84); rows of A^-1.
+            const int64_t ub = a.u.base(p);
+            T am[9], ai[9];
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+#pragma unroll
+                for (int e = 0; e < 3; ++e)
+                    am[r * 3 + e] = (r == e ? T(1) : T(0)) + dt * a.u.at(ub, (1 + r) * 3 + e);
+            inverse3(am, det3(am), ai);
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                v.x = ai[r * 3 + 0];
+                v.y = ai[r * 3 + 1];
+                v.z = ai[r * 3 + 2];
+                v.w = T(0);
+                *reinterpret_cast<T4*>(lds + stage_at(q, 1 + r)) = v;
+            }
+            pidx[q] = uint32_t(p);
+        }
+        __syncthreads();
+        const uint32_t s1 = min(b1, c1);
+        for (uint32_t s = max(b0, c0); s < s1; ++s) {
+            const int q = int(s - c0);
+            const T4 fm = *reinterpret_cast<const T4*>(lds + stage_at(q, 0));
+            if (fm.x == T(STALE_F)) continue;
+            T ai[3][3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                const T4 row = *reinterpret_cast<const T4*>(lds + stage_at(q, 1 + r));
+                ai[r][0] = row.x;
+                ai[r][1] = row.y;
+                ai[r][2] = row.z;
+            }
+            // Coefficients of this work-item's component: the modes of
+            // column d of c_p, or the constant 1 for the mass.
+            T C[E][E][E];
+#pragma unroll
+            for (int e0 = 0; e0 < E; ++e0)
+#pragma unroll
+                for (int e1 = 0; e1 < E; ++e1)
+#pragma unroll
+                    for (int e2 = 0; e2 < E; ++e2) C[e0][e1][e2] = T(0);
+            const int64_t cb = a.c.base(int64_t(pidx[q]));
+#pragma unroll
+            for (int r = 0; r < NM; ++r) {
+                const T cr = a.c.at(cb, r * 3 + d);
+                C[MODE_EXP[r][0]][MODE_EXP[r][1]][MODE_EXP[r][2]] =
+                    comp == 0 ? (r == 0 ? T(1) : T(0)) : cr;
+            }
+            const T fr[3] = {fm.x, fm.y, fm.z};
+            Spline<ORDER, T> sd;
+            spline_from_frac<ORDER, T, false, true>(a.lat, fr, sd);
+            // y = A^-1 d_n = P_i + Q_j + R_k.
+            T P[NK][3], Q[NK][3], R[NK][3];
+#pragma unroll
+            for (int n = 0; n < NK; ++n)
+#pragma unroll
+                for (int r = 0; r < 3; ++r) {
+                    P[n][r] = ai[r][0] * sd.d[0][n];
+                    Q[n][r] = ai[r][1] * sd.d[1][n];
+                    R[n][r] = ai[r][2] * sd.d[2][n];
+                }
+#pragma unroll
+            for (int i = 0; i < NK; ++i) {
+                const T wim = fm.w * sd.w[0][i];
+#pragma unroll
+                for (int j = 0; j < NK; ++j) {
+                    const T wij = wim * sd.w[1][j];
+                    const T pq[3] = {P[i][0] + Q[j][0], P[i][1] + Q[j][1], P[i][2] + Q[j][2]};
+#pragma unroll
+                    for (int k = 0; k < NK; ++k) {
+                        const T y[3] = {pq[0] + R[k][0], pq[1] + R[k][1], pq[2] + R[k][2]};
+                        acc[(i * NK + j) * NK + k] += (wij * sd.w[2][k]) * nested_horner<E>(C, y);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    for (int r = 0; r < S::NROUND; ++r) {
+        const int r_lo = r * S::CPR;
+        if (comp >= r_lo && comp < r_lo + S::CPR) {
+            T* dst = lds + l * S::CPR + (comp - r_lo);
+#pragma unroll
+            for (int n = 0; n < NN; ++n) dst[n * BINS * S::CPR] = acc[n];
+        }
+        __syncthreads();
+        gather_flush<NK, PB, NT, S::CPR>(lds, f0, r_lo, 1, a.gm, a.gmu, a.err);
+        __syncthreads();
+    }
+}
+
+template <int ORDER, typename T>
+__global__ __launch_bounds__((PolyBrickShape<ORDER, T>::NT)) void k_polypic_p2g_brick(
+    PolyBrickArgs<T> g)
+{
+    using S = PolyBrickShape<ORDER, T>;
+    __shared__ __align__(16) T lds[S::LDS];
+    __shared__ uint32_t pidx[S::CAP];
+    __shared__ uint32_t rng[2 * S::BINS];
+    __shared__ uint32_t slot[2];
+    walk_bricks<true>(g, slot, [&](uint32_t b) {
+        polypic_p2g_one_brick<ORDER, T>(g, b, lds, pidx, rng);
+    });
+}
+
+template <typename T>
+static int polypic_p2g_brick_run(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
+                                 int64_t np, const pic_pfield* x, const pic_pfield* m,
+                                 const pic_pfield* u, const pic_pfield* c, const pic_grid& gmu,
+                                 const pic_grid* gm, double dt, const pic_bins& bins);
+
 template <typename T>
 struct PolyLaunch {
     PolyArgs<T> a;
@@ -183,7 +425,7 @@ struct PolyLaunch {
     template <int O, int NC, bool S>
     void go()
     {
-        const unsigned b = blocks_for(a.np, 256);
+        const unsigned b = a.list ? 512u : blocks_for(a.np, 256);
         if (p2g) k_polypic_p2g<O, NC, S, T><<<b, 256, 0, st>>>(a);
         else k_polypic_g2p<O, NC, S, T><<<b, 256, 0, st>>>(a);
     }
@@ -213,6 +455,7 @@ static PolyArgs<T> make_poly_args(pic_context* ctx, int entity, int ncomp, const
     a.lat = make_lattice(mesh, entity);
     a.np = np;
     a.perm = perm;
+    a.list = nullptr;
     a.x = make_pfield<T>(*x);
     a.m = m ? make_pfield<T>(*m) : PField<T>{};
     a.u = u ? make_pfield<T>(*u) : PField<T>{};
@@ -227,6 +470,35 @@ static PolyArgs<T> make_poly_args(pic_context* ctx, int entity, int ncomp, const
     a.dt = dt;
     a.err = ctx->d_err;
     return a;
+}
+
+template <typename T>
+static int polypic_p2g_brick_run(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
+                                 int64_t np, const pic_pfield* x, const pic_pfield* m,
+                                 const pic_pfield* u, const pic_pfield* c, const pic_grid& gmu,
+                                 const pic_grid* gm, double dt, const pic_bins& bins)
+{
+    if (brick_edge(bins.geom) != 4) return PIC_ERR_UNSUPPORTED;
+    PolyBrickArgs<T> g;
+    if (!setup_walk(ctx, order, entity, np, bins, g)) return PIC_ERR_UNSUPPORTED;
+    g.a = make_poly_args<T>(ctx, entity, 3, mesh, np, x, m, u, c, gmu, gm, dt, bins.perm);
+    {
+        TimedLaunch tl(ctx, "polypic_p2g");
+        auto go = [&](auto kernel, int nt) {
+            const unsigned nbk = persistent_blocks(ctx, kernel, nt, 0, g.nbricks);
+            kernel<<<nbk, nt, 0, ctx->stream>>>(g);
+        };
+        if (order == 1) go(k_polypic_p2g_brick<1, T>, PolyBrickShape<1, T>::NT);
+        else if (order == 2) go(k_polypic_p2g_brick<2, T>, PolyBrickShape<2, T>::NT);
+        else go(k_polypic_p2g_brick<3, T>, PolyBrickShape<3, T>::NT);
+        if (launch_status()) return PIC_ERR_HIP;
+    }
+    // Particles the brick kernel skipped (stale bins).
+    TimedLaunch tl(ctx, "polypic_p2g_stale");
+    PolyLaunch<T> L{g.a, ctx->stream, true};
+    L.a.list = g.stale;
+    L.run(order, 3, false);
+    return launch_status();
 }
 
 }  // namespace pic
@@ -267,6 +539,14 @@ extern "C" int pic_polypic_p2g(pic_context* ctx, int order, int entity, int dtyp
         if (rc) return rc;
     }
     if (np == 0) return PIC_OK;
+    if (bins && !stag && ncomp == 3) {
+        const int rc = dtype == PIC_F64
+                           ? polypic_p2g_brick_run<double>(ctx, order, entity, *mesh, np, x, m_p,
+                                                           u_p, c_p, *mu_i, m_i, dt, *bins)
+                           : polypic_p2g_brick_run<float>(ctx, order, entity, *mesh, np, x, m_p,
+                                                          u_p, c_p, *mu_i, m_i, dt, *bins);
+        if (rc != PIC_ERR_UNSUPPORTED) return rc;
+    }
     const uint32_t* perm = bins ? bins->perm : nullptr;
     TimedLaunch tl(ctx, "polypic_p2g");
     if (dtype == PIC_F64) {

@@ -53,9 +53,12 @@ __device__ __forceinline__ uint32_t wave_agg_inc(uint32_t* base, uint32_t key, b
     return slot;
 }
 
+// Particles outside the bins are counted in count[nbricks] (they are placed
+// after every brick, so perm stays a permutation even then).
 template <typename T>
 __global__ __launch_bounds__(256) void k_cell_keys(int64_t np, PField<T> x, KeyGeom g,
-                                                   uint32_t* keys, uint32_t* count, int* err)
+                                                   uint32_t nbricks, uint32_t* keys,
+                                                   uint32_t* count, int* err)
 {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool live = p < np;
@@ -63,9 +66,12 @@ __global__ __launch_bounds__(256) void k_cell_keys(int64_t np, PField<T> x, KeyG
     if (live) {
         key = cell_key(g, x, p, &b);
         keys[p] = key;
-        if (key == 0xffffffffu) atomicOr(err, ERRBIT_CELL);
+        if (key == 0xffffffffu) {
+            atomicOr(err, ERRBIT_CELL);
+            b = nbricks;
+        }
     }
-    (void)wave_agg_inc(count, b, live && key != 0xffffffffu);
+    (void)wave_agg_inc(count, b, live);
 }
 
 // ---------------------------------------------------------------------------
@@ -148,17 +154,25 @@ void exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int64_t n
 }
 
 // ---------------------------------------------------------------------------
+// Particles outside the bins go straight to their final slots after the
+// last brick (key 0xffffffff, order unspecified: the call reports an error).
 __global__ __launch_bounds__(256) void k_bin_scatter(int64_t np, const uint32_t* keys,
-                                                     uint32_t bvol, const uint32_t* start,
-                                                     uint32_t* cursor, uint32_t* tmp)
+                                                     uint32_t bvol, uint32_t nbricks,
+                                                     const uint32_t* start, uint32_t* cursor,
+                                                     uint32_t* tmp, uint32_t* perm,
+                                                     uint32_t* keys_out)
 {
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool live = p < np;
     const uint32_t key = live ? keys[p] : 0xffffffffu;
-    const bool ok = live && key != 0xffffffffu;
-    const uint32_t b = ok ? key / bvol : 0u;
-    const uint32_t r = wave_agg_inc(cursor, b, ok);
-    if (ok) tmp[start[b] + r] = uint32_t(p);
+    const bool ok = key != 0xffffffffu;
+    const uint32_t b = ok ? key / bvol : nbricks;
+    const uint32_t r = wave_agg_inc(cursor, b, live);
+    if (live && ok) tmp[start[b] + r] = uint32_t(p);
+    if (live && !ok) {
+        perm[start[nbricks] + r] = uint32_t(p);
+        keys_out[start[nbricks] + r] = 0xffffffffu;
+    }
 }
 
 // Sort one brick segment held in `items` (LDS or global scratch): count the
@@ -304,19 +318,19 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
     DeviceGuard dg(ctx->device);
     if (!dg.ok) return PIC_ERR_HIP;
 
-    // scratch: count[nb+1] | cursor[nb] | scan tmp | act[nb+1] | keys_in[np] | tmp[w] | big[w]
+    // scratch: count[nb+1] | cursor[nb+1] | scan tmp | act[nb+1] | keys_in[np] | tmp[w] | big[w]
     // with w = max(np, nb+1) (tmp and big also hold the non-empty flags and
     // their scan before the scatter).  act holds the active-brick list when
     // the caller does not ask for it.
     const int64_t scan_words = scan_tmp_words(nb + 1);
-    const int64_t head = (nb + 1) + nb + scan_words + (active ? 0 : nb + 1);
+    const int64_t head = (nb + 1) + (nb + 1) + scan_words + (active ? 0 : nb + 1);
     const int64_t w = std::max<int64_t>(np, nb + 1);
     const size_t bytes = size_t(head + np + 2 * w) * 4;
     uint32_t* sc = static_cast<uint32_t*>(scratch(ctx, bytes));
     if (!sc) return PIC_ERR_ALLOC;
     uint32_t* count = sc;
     uint32_t* cursor = count + nb + 1;
-    uint32_t* scan_tmp = cursor + nb;
+    uint32_t* scan_tmp = cursor + nb + 1;
     uint32_t* act = active ? active : scan_tmp + scan_words;
     uint32_t* keys_in = sc + head;
     uint32_t* tmp = keys_in + np;
@@ -324,15 +338,17 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
     hipStream_t st = ctx->stream;
 
     TimedLaunch tl(ctx, "bin_sort");
-    const unsigned fb = (unsigned)std::min<int64_t>((2 * nb + 1 + 255) / 256, 4096);
-    k_fill_u32<<<fb, 256, 0, st>>>(count, 2 * nb + 1, 0u);
+    const unsigned fb = (unsigned)std::min<int64_t>((2 * nb + 2 + 255) / 256, 4096);
+    k_fill_u32<<<fb, 256, 0, st>>>(count, 2 * nb + 2, 0u);
     if (np > 0) {
         const unsigned b = blocks_for(np, 256);
         if (dtype == PIC_F64)
-            k_cell_keys<double><<<b, 256, 0, st>>>(np, make_pfield<double>(*x), g, keys_in, count,
+            k_cell_keys<double><<<b, 256, 0, st>>>(np, make_pfield<double>(*x), g, uint32_t(nb),
+                                                    keys_in, count,
                                                     ctx->d_err);
         else
-            k_cell_keys<float><<<b, 256, 0, st>>>(np, make_pfield<float>(*x), g, keys_in, count,
+            k_cell_keys<float><<<b, 256, 0, st>>>(np, make_pfield<float>(*x), g, uint32_t(nb),
+                                                   keys_in, count,
                                                    ctx->d_err);
     }
     exclusive_scan(st, count, bin_start, nb + 1, scan_tmp);
@@ -347,8 +363,8 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
         k_compact_active<<<fb2, 256, 0, st>>>(flags, pos, nb, act);
     }
     if (np > 0) {
-        k_bin_scatter<<<blocks_for(np, 256), 256, 0, st>>>(np, keys_in, g.bvol, bin_start, cursor,
-                                                            tmp);
+        k_bin_scatter<<<blocks_for(np, 256), 256, 0, st>>>(np, keys_in, g.bvol, uint32_t(nb),
+                                                            bin_start, cursor, tmp, perm, keys);
         const size_t lds = size_t(2 * g.bvol + SORT_LDS_CAP) * 4;
         const unsigned nbk = persistent_blocks(ctx, k_brick_sort, SORT_BT, lds, nb);
         k_brick_sort<<<nbk, SORT_BT, lds, st>>>(bin_start, act, tmp, keys_in, g.bvol, big, perm,

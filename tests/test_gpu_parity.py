@@ -176,12 +176,27 @@ def test_bin_sort_bit_exact(pa, wl0, brick, dt, dual):
 
 
 def test_bin_sort_out_of_range_reported(pa, wl0):
+    """Particles outside the bins are reported; perm stays a permutation,
+    with those particles after the last brick (key 0xffffffff) and the
+    others sorted as the oracle sorts them."""
     import gpu_util as gu
     x = gu.soa(wl0.x)
     geom = pa.Binning((3, 2, 2), (31, 32, 32), (8, 8, 8), 0)  # cells at i = 2 left out
-    pa.bin_sort(gu.ctx(), pa.Mesh(wl0.low, wl0.h), x, geom)
+    res = pa.bin_sort(gu.ctx(), pa.Mesh(wl0.low, wl0.h), x, geom)
     with pytest.raises(pa.PicError):
         gu.ctx().synchronize()
+    perm = gu.host(res.perm).view(np.uint32)
+    keys = gu.host(res.keys).view(np.uint32)
+    np.testing.assert_array_equal(np.sort(perm), np.arange(len(wl0.x), dtype=np.uint32))
+    cells = np.floor((wl0.x - wl0.low) / wl0.h).astype(np.int64)
+    outside = cells[:, 0] < 3
+    nin = int((~outside).sum())
+    assert outside.any()
+    assert outside[perm[nin:]].all() and (keys[nin:] == 0xFFFFFFFF).all()
+    ref_keys, _ = orpic.cell_keys(wl0.x[~outside], wl0.low, wl0.h, geom.cell_lo, geom.ncell,
+                                  (8, 8, 8), 0)
+    ref_perm, _ = orpic.bin_sort(ref_keys, pa.bin_count(geom) * 512)
+    np.testing.assert_array_equal(perm[:nin], np.flatnonzero(~outside)[ref_perm])
 
 
 def _check_brick(pa, wl, order, entity, dtype, edge):
@@ -519,3 +534,58 @@ def test_g2p_update_F_matches_oracle(pa, wl0, order, dtype, binned, fused_norm):
     assert gu.rel_inf(gu.host(F).reshape(-1, 3, 3), rF) < ftol
     assert gu.rel_inf(gu.host(det)[:, 0], rdet) < ftol
     assert gu.rel_inf(gu.host(Finv).reshape(-1, 3, 3), rFinv) < ftol
+
+
+def _polypic_binned_check(pa, wl, order, dtype, stale=False):
+    """PolyPIC p2g through the brick kernel (bins from pic_bin_sort) against
+    the oracle, three grid components."""
+    import gpu_util as gu
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    f32 = dtype == "f32"
+    rng = np.random.default_rng(90 + order)
+    P = len(wl.x)
+    nm = orpic.polypic_nmode(order)
+    x = wl.x.copy()
+    m = wl.m.copy()
+    u = rng.normal(0, 0.5, size=(P, nm, 3))
+    cm = rng.normal(size=(P, nm, 3))
+    if f32:
+        x, m, u, cm = (a.astype(np.float32).astype(np.float64) for a in (x, m, u, cm))
+    lo, n = wl.node_window
+    mesh = pa.Mesh(wl.low, wl.h)
+    xt = gu.soa(x, tdt)
+    bins = pa.bin_sort(gu.ctx(), mesh, xt, _binning(pa, wl, order, 0, (4, 4, 4)))
+    if stale:
+        gu.ctx().synchronize()
+        sel = rng.random(P) < 0.3
+        x[sel] += rng.uniform(-1.5, 1.5, size=(sel.sum(), 3)) * wl.h
+        x = np.clip(x, 0.0, 1.0 - 1e-9)
+        if f32:
+            x = x.astype(np.float32).astype(np.float64)
+        xt.copy_(gu.soa(x, tdt))
+    dt = 1e-3
+    rmu = np.zeros(tuple(n) + (3,))
+    rm = np.zeros(tuple(n))
+    orpic.polypic_p2g(order, orpic.NODE, wl.low, wl.h, lo, n, x, m, u, cm, dt, rmu, rm)
+    sd = pa.create_spline(0, order, mesh, xt, bins)
+    gmu = pa.GridField(gu.dev(np.zeros(rmu.shape), tdt))
+    gm = pa.GridField(gu.dev(np.zeros(rm.shape), tdt))
+    pa.PolyPIC.p2g(gu.ctx(), gu.soa(m[:, None], tdt), gu.soa(u.reshape(P, -1), tdt),
+                   gu.soa(cm.reshape(P, -1), tdt), gmu, gm, dt, sd, zero_first=True)
+    gu.ctx().synchronize()
+    tol = F32_TOL if f32 else F64_TOL
+    assert gu.rel_inf(gu.host(gm.tensor), rm) < tol
+    assert gu.rel_inf(gu.host(gmu.tensor), rmu) < tol
+    assert abs(gu.host(gm.tensor).sum() - m.sum()) <= (1e-6 if f32 else 1e-13) * m.sum()
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_polypic_brick_p2g_vs_oracle(pa, wl0, order, dtype):
+    _polypic_binned_check(pa, wl0, order, dtype)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_polypic_brick_p2g_stale_and_dense(pa, wl0, wl_dense, dtype):
+    _polypic_binned_check(pa, wl0, 3, dtype, stale=True)
+    _polypic_binned_check(pa, wl_dense, 3, dtype)
