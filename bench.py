@@ -270,7 +270,7 @@ def config_polypic(pa, ctx, synth, steps=5):
                             "max_mass", "polypic_g2p"])
     ctx.reset_timing()
     return {"workload": "PolyPIC uniform 128^3 cells x 8 ppc, cubic node spline, 27 modes x 3, "
-                        "FP32, cell-bin sort every step, brick p2g, per-particle g2p",
+                        "FP32, cell-bin sort every step, brick p2g and g2p",
             "particles": P, "ms_per_step": ms, "particles_per_s": P / (ms * 1e-3),
             "kernels_ms": kt}
 

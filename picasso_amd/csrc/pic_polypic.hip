@@ -130,13 +130,24 @@ __device__ __forceinline__ void polypic_p2g_particle(const PolyArgs<T>& a, int64
 
 // PolyPIC::g2p (Picasso_PolyPIC.hpp:235-293, 299-359).
 template <int ORDER, int NC, bool STAG, typename T>
+__device__ __forceinline__ void polypic_g2p_particle(const PolyArgs<T>& a, int64_t p);
+
+template <int ORDER, int NC, bool STAG, typename T>
 __global__ __launch_bounds__(256) void k_polypic_g2p(PolyArgs<T> a)
+{
+    const int64_t n = a.list ? int64_t(a.list[0]) : a.np;
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t s = a.list ? int64_t(a.list[1 + i]) : i;
+        polypic_g2p_particle<ORDER, NC, STAG, T>(a, a.perm ? int64_t(a.perm[s]) : s);
+    }
+}
+
+template <int ORDER, int NC, bool STAG, typename T>
+__device__ __forceinline__ void polypic_g2p_particle(const PolyArgs<T>& a, int64_t p)
 {
     constexpr int NK = Knots<ORDER>::NK;
     constexpr int NM = Modes<ORDER>::NM;
-    const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (s >= a.np) return;
-    const int64_t p = a.perm ? int64_t(a.perm[s]) : s;
     const int64_t xb = a.x.base(p);
     T xp[3] = {a.x.at(xb, 0), a.x.at(xb, 1), a.x.at(xb, 2)};
     Spline<ORDER, T> sd;
@@ -397,6 +408,125 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
     }
 }
 
+// Brick g2p (collocated, three components): the brick's (4+ORDER)^3 grid
+// footprint is staged in LDS once, then one work-item per particle forms
+// its modes, the Taylor coefficients sum_n F0_e0(i) F1_e1(j) F2_e2(k) u_n
+// (F_e = weight, gradient, half second derivative), summed one dimension
+// at a time (k, then j, then i) instead of per node and mode.
+template <int ORDER, typename T>
+__device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g, uint32_t b,
+                                                      T* us)
+{
+    constexpr int PB = 4, NT = 256;
+    constexpr int NK = ORDER + 1;
+    constexpr int FN = PB + ORDER;
+    constexpr int FV = FN * FN * FN;
+    constexpr int NM = Modes<ORDER>::NM;
+    constexpr int E = ORDER == 1 ? 2 : 3;
+    const PolyArgs<T>& a = g.a;
+    const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
+    int f0[3];
+    brick_origin<PB>(g, b, f0);
+    const GView<T>& gu = a.gmu;
+    for (int q = threadIdx.x; q < FV; q += NT) {
+        const int ni = f0[0] + q / (FN * FN), nj = f0[1] + (q / FN) % FN, nk = f0[2] + q % FN;
+        const bool in = gu.covers(0, ni, ni) && gu.covers(1, nj, nj) && gu.covers(2, nk, nk);
+        const int64_t off = in ? gu.off(ni, nj, nk) : 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) us[d * FV + q] = in ? gu.data[off + d * gu.cs] : T(0);
+    }
+    __syncthreads();
+    for (uint32_t s = beg + threadIdx.x; s < end; s += NT) {
+        const int64_t p = int64_t(a.perm[s]);
+        const int64_t xb = a.x.base(p);
+        const T xp[3] = {a.x.at(xb, 0), a.x.at(xb, 1), a.x.at(xb, 2)};
+        Spline<ORDER, T> sd;
+        eval_spline<ORDER, T, true, false, (NM > 8)>(a.lat, xp, sd);
+        const int li = sd.s0[0] - f0[0], lj = sd.s0[1] - f0[1], lk = sd.s0[2] - f0[2];
+        if (!(li >= 0 && li < PB && lj >= 0 && lj < PB && lk >= 0 && lk < PB)) {
+            push_stale(g, s);
+            continue;
+        }
+        // F[q][e](n): weight, gradient, half second derivative of dim q.
+        T F[3][E][NK];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int n = 0; n < NK; ++n) {
+                F[q][0][n] = sd.w[q][n];
+                F[q][1][n] = sd.g[q][n];
+                if constexpr (E > 2) F[q][2][n] = T(0.5) * sd.g2[q][n];
+            }
+        const int lbase = (li * FN + lj) * FN + lk;
+        T cm[E][E][E][3];
+#pragma unroll
+        for (int e0 = 0; e0 < E; ++e0)
+#pragma unroll
+            for (int e1 = 0; e1 < E; ++e1)
+#pragma unroll
+                for (int e2 = 0; e2 < E; ++e2)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) cm[e0][e1][e2][d] = T(0);
+#pragma unroll
+        for (int i = 0; i < NK; ++i) {
+            T t[E][E][3];
+#pragma unroll
+            for (int e1 = 0; e1 < E; ++e1)
+#pragma unroll
+                for (int e2 = 0; e2 < E; ++e2)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) t[e1][e2][d] = T(0);
+#pragma unroll
+            for (int j = 0; j < NK; ++j) {
+                T sk[E][3];
+#pragma unroll
+                for (int e2 = 0; e2 < E; ++e2)
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) sk[e2][d] = T(0);
+#pragma unroll
+                for (int k = 0; k < NK; ++k) {
+                    const int q = lbase + (i * FN + j) * FN + k;
+                    const T ue[3] = {us[q], us[FV + q], us[2 * FV + q]};
+#pragma unroll
+                    for (int e2 = 0; e2 < E; ++e2)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) sk[e2][d] += F[2][e2][k] * ue[d];
+                }
+#pragma unroll
+                for (int e1 = 0; e1 < E; ++e1)
+#pragma unroll
+                    for (int e2 = 0; e2 < E; ++e2)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) t[e1][e2][d] += F[1][e1][j] * sk[e2][d];
+            }
+#pragma unroll
+            for (int e0 = 0; e0 < E; ++e0)
+#pragma unroll
+                for (int e1 = 0; e1 < E; ++e1)
+#pragma unroll
+                    for (int e2 = 0; e2 < E; ++e2)
+#pragma unroll
+                        for (int d = 0; d < 3; ++d) cm[e0][e1][e2][d] += F[0][e0][i] * t[e1][e2][d];
+        }
+        const int64_t cb = a.c.base(p);
+#pragma unroll
+        for (int r = 0; r < NM; ++r)
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                a.c.at(cb, r * 3 + d) = cm[MODE_EXP[r][0]][MODE_EXP[r][1]][MODE_EXP[r][2]][d];
+    }
+    __syncthreads();
+}
+
+template <int ORDER, typename T>
+__global__ __launch_bounds__(256) void k_polypic_g2p_brick(PolyBrickArgs<T> g)
+{
+    constexpr int FN = 4 + ORDER;
+    __shared__ T us[3 * FN * FN * FN];
+    __shared__ uint32_t slot[2];
+    walk_bricks<false>(g, slot, [&](uint32_t b) { polypic_g2p_one_brick<ORDER, T>(g, b, us); });
+}
+
 template <int ORDER, typename T>
 __global__ __launch_bounds__((PolyBrickShape<ORDER, T>::NT)) void k_polypic_p2g_brick(
     PolyBrickArgs<T> g)
@@ -501,6 +631,34 @@ static int polypic_p2g_brick_run(pic_context* ctx, int order, int entity, const 
     return launch_status();
 }
 
+template <typename T>
+static int polypic_g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
+                                 int64_t np, const pic_pfield* x, const pic_pfield* c,
+                                 const pic_grid& gu, const pic_bins& bins)
+{
+    if (brick_edge(bins.geom) != 4) return PIC_ERR_UNSUPPORTED;
+    PolyBrickArgs<T> g;
+    if (!setup_walk(ctx, order, entity, np, bins, g)) return PIC_ERR_UNSUPPORTED;
+    g.a = make_poly_args<T>(ctx, entity, 3, mesh, np, x, nullptr, nullptr, c, gu, nullptr, 0.0,
+                            bins.perm);
+    {
+        TimedLaunch tl(ctx, "polypic_g2p");
+        auto go = [&](auto kernel) {
+            const unsigned nbk = persistent_blocks(ctx, kernel, 256, 0, g.nbricks);
+            kernel<<<nbk, 256, 0, ctx->stream>>>(g);
+        };
+        if (order == 1) go(k_polypic_g2p_brick<1, T>);
+        else if (order == 2) go(k_polypic_g2p_brick<2, T>);
+        else go(k_polypic_g2p_brick<3, T>);
+        if (launch_status()) return PIC_ERR_HIP;
+    }
+    TimedLaunch tl(ctx, "polypic_g2p_stale");
+    PolyLaunch<T> L{g.a, ctx->stream, false};
+    L.a.list = g.stale;
+    L.run(order, 3, false);
+    return launch_status();
+}
+
 }  // namespace pic
 
 using namespace pic;
@@ -578,6 +736,14 @@ extern "C" int pic_polypic_g2p(pic_context* ctx, int order, int entity, int dtyp
     DeviceGuard g(ctx->device);
     if (!g.ok) return PIC_ERR_HIP;
     const bool stag = stag_dim(entity) >= 0;
+    if (bins && !stag && ncomp == 3) {
+        const int rc = dtype == PIC_F64
+                           ? polypic_g2p_brick_run<double>(ctx, order, entity, *mesh, np, x, c_p,
+                                                           *u_i, *bins)
+                           : polypic_g2p_brick_run<float>(ctx, order, entity, *mesh, np, x, c_p,
+                                                          *u_i, *bins);
+        if (rc != PIC_ERR_UNSUPPORTED) return rc;
+    }
     const uint32_t* perm = bins ? bins->perm : nullptr;
     TimedLaunch tl(ctx, "polypic_g2p");
     if (dtype == PIC_F64) {

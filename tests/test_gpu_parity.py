@@ -572,11 +572,21 @@ def _polypic_binned_check(pa, wl, order, dtype, stale=False):
     gm = pa.GridField(gu.dev(np.zeros(rm.shape), tdt))
     pa.PolyPIC.p2g(gu.ctx(), gu.soa(m[:, None], tdt), gu.soa(u.reshape(P, -1), tdt),
                    gu.soa(cm.reshape(P, -1), tdt), gmu, gm, dt, sd, zero_first=True)
+    gfield = rng.normal(size=tuple(n) + (3,))
+    if f32:
+        gfield = gfield.astype(np.float32).astype(np.float64)
+    rc = orpic.polypic_g2p(order, orpic.NODE, wl.low, wl.h, lo, n, x, gfield, 3)
+    cg = gu.soa(np.zeros((P, nm * 3)), tdt)
+    pa.PolyPIC.g2p(gu.ctx(), pa.GridField(gu.dev(gfield, tdt)), cg, sd)
     gu.ctx().synchronize()
     tol = F32_TOL if f32 else F64_TOL
     assert gu.rel_inf(gu.host(gm.tensor), rm) < tol
     assert gu.rel_inf(gu.host(gmu.tensor), rmu) < tol
     assert abs(gu.host(gm.tensor).sum() - m.sum()) <= (1e-6 if f32 else 1e-13) * m.sum()
+    cgh = gu.host(cg).reshape(P, nm, 3)
+    # per mode family (values, first and second derivatives differ in scale)
+    for lo_r, hi_r in ((0, 1), (1, 4), (4, nm)):
+        assert gu.rel_inf(cgh[:, lo_r:hi_r], rc[:, lo_r:hi_r]) < tol * 10
 
 
 @pytest.mark.parametrize("order", [1, 2, 3])
