@@ -256,6 +256,21 @@ int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh,
                  uint32_t* keys, uint32_t* perm, uint32_t* bin_start,
                  uint32_t* active);
 
+/* Step graphs (no reference counterpart: Kokkos launches each kernel).
+ * Between pic_graph_begin and pic_graph_end the context's calls are
+ * captured into a HIP graph instead of running; pic_graph_launch replays
+ * the whole sequence on the context's stream with one launch.  The context
+ * must be bound to a stream other than the null stream.  A captured call
+ * may not need more scratch than the context already holds (run the same
+ * calls once before capturing); such a call returns PIC_ERR_ALLOC.  No
+ * kernel timing is recorded inside a capture; pic_synchronize and a change
+ * of stream are refused until pic_graph_end. */
+typedef struct pic_graph pic_graph;
+int pic_graph_begin(pic_context* ctx);
+int pic_graph_end(pic_context* ctx, pic_graph** graph);
+int pic_graph_launch(pic_context* ctx, const pic_graph* graph);
+int pic_graph_destroy(pic_graph* graph);
+
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on
  * the context's stream and pic_kernel_time() returns the accumulated
  * milliseconds and launch count of the named kernel family

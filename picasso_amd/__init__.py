@@ -94,9 +94,44 @@ class Context:
               "pic_kernel_time")
         return ms.value, n.value
 
+    def graph_begin(self):
+        """Start capturing this context's calls (on torch's current stream,
+        which must not be the null stream) into a step graph."""
+        self._bind_stream()
+        check(lib.pic_graph_begin(self.handle), "pic_graph_begin")
+
+    def graph_end(self):
+        """Finish the capture; returns a StepGraph to replay."""
+        g = ctypes.c_void_p()
+        check(lib.pic_graph_end(self.handle, ctypes.byref(g)), "pic_graph_end")
+        return StepGraph(self, g)
+
     def close(self):
         if getattr(self, "handle", None):
             lib.pic_context_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class StepGraph:
+    """A captured sequence of calls (pic_graph); launch() replays it on the
+    context's current stream."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def launch(self):
+        check(lib.pic_graph_launch(self.ctx._bind_stream(), self.handle), "pic_graph_launch")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.pic_graph_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

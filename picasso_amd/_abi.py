@@ -87,6 +87,10 @@ PROTOTYPES = {
     "pic_apic_g2p_update_F": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(Mesh), _c_i64,
                                        _P(PField), _P(Grid), _P(Grid), _P(PField), _c_double,
                                        _P(PField), _P(PField), _P(PField), _P(Bins)]),
+    "pic_graph_begin": (_c_int, [_vp]),
+    "pic_graph_end": (_c_int, [_vp, _P(_vp)]),
+    "pic_graph_launch": (_c_int, [_vp, _vp]),
+    "pic_graph_destroy": (_c_int, [_vp]),
     "pic_set_timing": (_c_int, [_vp, _c_int]),
     "pic_kernel_time": (_c_int, [_vp, ctypes.c_char_p, _P(_c_double), _P(_c_i64)]),
     "pic_reset_timing": (_c_int, [_vp]),

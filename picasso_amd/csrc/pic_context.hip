@@ -9,6 +9,7 @@ namespace pic {
 void* scratch(pic_context* ctx, size_t bytes)
 {
     if (bytes <= ctx->scratch_bytes) return ctx->scratch;
+    if (ctx->capturing) return nullptr;  // no allocation inside a graph capture
     // The stream may still use the old buffer.
     if (ctx->scratch) {
         if (hipStreamSynchronize(ctx->stream) != hipSuccess) return nullptr;
@@ -39,7 +40,7 @@ static hipEvent_t take_event(pic_context* ctx)
 
 TimedLaunch::TimedLaunch(pic_context* c, const char* name) : ctx(c), idx(SIZE_MAX)
 {
-    if (!ctx->timing) return;
+    if (!ctx->timing || ctx->capturing) return;
     pic_context::Rec r{name, take_event(ctx), take_event(ctx)};
     if (!r.start || !r.stop) return;
     (void)hipEventRecord(r.start, ctx->stream);
@@ -206,13 +207,14 @@ int pic_context_destroy(pic_context* ctx)
 int pic_set_stream(pic_context* ctx, void* stream)
 {
     if (!ctx) return PIC_ERR_ARG;
+    if (ctx->capturing) return static_cast<hipStream_t>(stream) == ctx->stream ? PIC_OK : PIC_ERR_ARG;
     ctx->stream = static_cast<hipStream_t>(stream);
     return PIC_OK;
 }
 
 int pic_synchronize(pic_context* ctx)
 {
-    if (!ctx) return PIC_ERR_ARG;
+    if (!ctx || ctx->capturing) return PIC_ERR_ARG;
     DeviceGuard g(ctx->device);
     PIC_HIP_CHECK(hipStreamSynchronize(ctx->stream));
     int err = 0;
@@ -265,3 +267,62 @@ int pic_kernel_time(pic_context* ctx, const char* name, double* ms, int64_t* lau
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Step graphs: capture of the context's calls into a HIP graph.
+struct pic_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int device = 0;
+};
+
+int pic_graph_begin(pic_context* ctx)
+{
+    if (!ctx || ctx->capturing || !ctx->stream) return PIC_ERR_ARG;
+    DeviceGuard g(ctx->device);
+    if (!g.ok) return PIC_ERR_HIP;
+    PIC_HIP_CHECK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    return PIC_OK;
+}
+
+int pic_graph_end(pic_context* ctx, pic_graph** out)
+{
+    if (!ctx || !ctx->capturing || !out) return PIC_ERR_ARG;
+    *out = nullptr;
+    DeviceGuard g(ctx->device);
+    ctx->capturing = false;
+    if (!g.ok) return PIC_ERR_HIP;
+    hipGraph_t graph = nullptr;
+    PIC_HIP_CHECK(hipStreamEndCapture(ctx->stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return PIC_ERR_HIP;
+    }
+    pic_graph* pg = new pic_graph();
+    pg->graph = graph;
+    pg->exec = exec;
+    pg->device = ctx->device;
+    *out = pg;
+    return PIC_OK;
+}
+
+int pic_graph_launch(pic_context* ctx, const pic_graph* graph)
+{
+    if (!ctx || !graph || ctx->capturing || graph->device != ctx->device) return PIC_ERR_ARG;
+    DeviceGuard g(ctx->device);
+    if (!g.ok) return PIC_ERR_HIP;
+    PIC_HIP_CHECK(hipGraphLaunch(graph->exec, ctx->stream));
+    return PIC_OK;
+}
+
+int pic_graph_destroy(pic_graph* graph)
+{
+    if (!graph) return PIC_ERR_ARG;
+    DeviceGuard g(graph->device);
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    delete graph;
+    return PIC_OK;
+}

@@ -33,6 +33,10 @@ struct pic_context {
     std::vector<Rec> recs;
     std::vector<hipEvent_t> event_pool;
 
+    // Between pic_graph_begin and pic_graph_end: calls are being captured
+    // (no scratch growth, no timing events, no synchronisation).
+    bool capturing = false;
+
     // Resident workgroups per compute unit, per (kernel, dynamic LDS bytes).
     std::map<std::pair<const void*, size_t>, int> occupancy;
 };

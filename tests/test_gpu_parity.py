@@ -599,3 +599,49 @@ def test_polypic_brick_p2g_vs_oracle(pa, wl0, order, dtype):
 def test_polypic_brick_p2g_stale_and_dense(pa, wl0, wl_dense, dtype):
     _polypic_binned_check(pa, wl0, 3, dtype, stale=True)
     _polypic_binned_check(pa, wl_dense, 3, dtype)
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_step_graph_replays_match_eager(pa, wl0, dtype):
+    """sort -> p2g -> fused normalise+g2p captured once (pic_graph_*) and
+    replayed three times, with eager copies and synchronisations between the
+    replays; every replay matches the eager step (atomic summation order
+    aside)."""
+    import gpu_util as gu
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    tol = 1e-13 if dtype == "f64" else 2e-6
+    lo, n = wl0.node_window
+    mesh = pa.Mesh(wl0.low, wl0.h)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        ctx = pa.Context(0)
+        x = gu.soa(wl0.x, tdt)
+        m = gu.soa(wl0.m[:, None], tdt)
+        c0 = gu.soa(wl0.c.reshape(-1, 12), tdt)
+        c = c0.clone()
+        geom = _binning(pa, wl0, 2, 0, (4, 4, 4))
+        bins = pa.bin_sort(ctx, mesh, x, geom)
+        gm, gmu = _node_grids(pa, lo, n, tdt)
+
+        def step():
+            pa.bin_sort(ctx, mesh, x, geom, out=bins)
+            sd = pa.create_spline(0, 2, mesh, x, bins)
+            pa.APIC.p2g(ctx, m, c, gm, gmu, sd, zero_first=True)
+            pa.APIC.g2p_normalized(ctx, gm, gmu, c, sd)
+
+        step()  # eager reference (also sizes the scratch)
+        ctx.synchronize()
+        ref = (gu.host(gm.tensor).copy(), gu.host(gmu.tensor).copy(), gu.host(c).copy())
+        ctx.graph_begin()
+        step()
+        graph = ctx.graph_end()
+        for _ in range(3):
+            c.copy_(c0)
+            gm.tensor.fill_(-1.0)
+            graph.launch()
+            ctx.synchronize()
+            assert gu.rel_inf(gu.host(gm.tensor), ref[0]) < tol
+            assert gu.rel_inf(gu.host(gmu.tensor), ref[1]) < tol
+            assert gu.rel_inf(gu.host(c), ref[2]) < tol * 10
+        graph.close()
+        ctx.close()
