@@ -275,6 +275,136 @@ def config_polypic(pa, ctx, synth, steps=5):
             "kernels_ms": kt}
 
 
+def config_batched(pa, ctx, synth, n_inst=512, steps=3, lanes=4):
+    """Independent instances: grid edge n_k = round(16 * 4^u_k), u_k ~ U(0, 1)
+    (16^3 to 64^3 cells, log-uniform), ppc_k ~ U{2..16}, fields as the 32^3
+    workload, FP32, quadratic node spline.  One step = for every instance:
+    sort -> p2g -> g2p (normalisation fused).  The instances are dealt,
+    largest first, to `lanes` streams with a context each (own scratch); each
+    lane's part of the step is captured once into a step graph
+    (pic_graph_*), and the lanes' graphs are replayed concurrently."""
+    import torch
+    dev = "cuda:0"
+    rs = np.random.default_rng(0x5EED0003)
+    ns = np.rint(16 * 4.0 ** rs.uniform(0, 1, n_inst)).astype(np.int64)
+    ppcs = rs.integers(2, 17, n_inst)
+    counts = ns ** 3 * ppcs
+    P = int(counts.sum())
+    offs = np.concatenate([[0], np.cumsum(counts)])
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0x5EED0003)
+    X = torch.empty((3, P), dtype=torch.float32, device=dev)
+    M = torch.empty((1, P), dtype=torch.float32, device=dev)
+    C = torch.empty((12, P), dtype=torch.float32, device=dev)
+    side = torch.cuda.Stream()
+    inst = []
+    with torch.cuda.stream(side):
+        for k in range(n_inst):
+            n, ppc, a, b = int(ns[k]), int(ppcs[k]), int(offs[k]), int(offs[k + 1])
+            h = 1.0 / n
+            cell = torch.arange(n ** 3, device=dev).repeat_interleave(ppc)
+            ijk = torch.stack([cell // (n * n), (cell // n) % n, cell % n]).to(torch.float64)
+            x = ((ijk + torch.rand((3, b - a), generator=gen, device=dev, dtype=torch.float64)) * h)
+            X[:, a:b] = x.to(torch.float32)
+            M[:, a:b] = h ** 3 / ppc
+            C[0:3, a:b] = torch.randn((3, b - a), generator=gen, device=dev)
+            C[3:, a:b] = torch.randn((9, b - a), generator=gen, device=dev) * 0.1
+            low = np.full(3, -synth.HALO * h)
+            mesh = pa.Mesh(low, h)
+            geom = pa.Binning((synth.HALO,) * 3, (n + 1,) * 3, (4, 4, 4), 1)
+            xv = X[:, a:b].t()
+            bins = pa.bin_sort(ctx, mesh, xv, geom)
+            ctx.synchronize()  # raises if a particle fell outside the bins
+            perm = bins.perm.long() + a
+            for t in (X, M, C):
+                t[:, a:b] = t[:, perm]
+            nn = n + 1 + 2 * synth.HALO
+            gm = pa.GridField(torch.zeros((nn, nn, nn), dtype=torch.float32, device=dev))
+            gmu = pa.GridField(torch.zeros((3, nn, nn, nn), dtype=torch.float32,
+                                           device=dev).permute(1, 2, 3, 0))
+            inst.append(dict(mesh=mesh, geom=geom, bins=bins, x=X[:, a:b].t(), m=M[:, a:b].t(),
+                             c=C[:, a:b].t(), gm=gm, gmu=gmu, np=b - a))
+        C0 = C.clone()
+    torch.cuda.synchronize()
+
+    def run(cx, d):
+        pa.bin_sort(cx, d["mesh"], d["x"], d["geom"], out=d["bins"])
+        sd = pa.create_spline(pa.FieldLocation.Node, 2, d["mesh"], d["x"], d["bins"])
+        pa.APIC.p2g(cx, d["m"], d["c"], d["gm"], d["gmu"], sd, zero_first=True)
+        pa.APIC.g2p_normalized(cx, d["gm"], d["gmu"], d["c"], sd)
+
+    # Eager pass on one stream: times every instance on its own.
+    with torch.cuda.stream(side):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in inst]
+        for d, (e0, e1) in zip(inst, ev):
+            e0.record()
+            run(ctx, d)
+            e1.record()
+        ctx.synchronize()
+        per_inst = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
+    # Deal instances to lanes, largest first, to the least loaded lane.
+    order = np.argsort(-counts)
+    load = np.zeros(lanes)
+    groups = [[] for _ in range(lanes)]
+    for k in order:
+        j = int(np.argmin(load))
+        groups[j].append(int(k))
+        load[j] += counts[k]
+    streams = [torch.cuda.Stream() for _ in range(lanes)]
+    ctxs = [pa.Context(0) for _ in range(lanes)]
+    # Eager pass per lane: sizes each context's scratch before the capture.
+    for cx, st, grp in zip(ctxs, streams, groups):
+        with torch.cuda.stream(st):
+            for k in grp:
+                run(cx, inst[k])
+    torch.cuda.synchronize()
+    C.copy_(C0)
+    torch.cuda.synchronize()
+    # One step graph per lane, replayed on the lane's own stream so the
+    # lanes run concurrently.
+    t0 = time.perf_counter()
+    graphs = []
+    for cx, st, grp in zip(ctxs, streams, groups):
+        with torch.cuda.stream(st):
+            cx.graph_begin()
+            for k in grp:
+                run(cx, inst[k])
+            graphs.append(cx.graph_end())
+    t_capture = time.perf_counter() - t0
+
+    def replay():
+        for g_, st in zip(graphs, streams):
+            with torch.cuda.stream(st):
+                g_.launch()
+
+    replay()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        replay()
+        for st in streams:  # a step ends when every lane has finished
+            side.wait_stream(st)
+        for st in streams:
+            st.wait_stream(side)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    for cx, g_ in zip(ctxs, graphs):
+        cx.synchronize()
+        g_.close()
+        cx.close()
+    return {"workload": f"{n_inst} APIC instances, 16^3-64^3 cells (log-uniform), 2-16 ppc, FP32, "
+                        f"quadratic node spline, sort -> p2g -> g2p per instance, {lanes} concurrent "
+                        "streams with one step graph each",
+            "instances": n_inst, "particles": P, "ms_per_step": ms,
+            "particles_per_s": P / (ms * 1e-3), "graph_capture_s": t_capture,
+            "instance_ms_eager": {"min": float(per_inst.min()),
+                                  "median": float(np.median(per_inst)),
+                                  "max": float(per_inst.max())},
+            "instance_particles": {"min": int(counts.min()), "median": int(np.median(counts)),
+                                   "max": int(counts.max())}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -403,7 +533,10 @@ def main():
         others["polypic_cubic27"] = config_polypic(pa, ctx, synth)
         log(f"  {others['polypic_cubic27']}")
         torch.cuda.empty_cache()
-        others["batched_instances"] = None  # batched multi-instance launch: not built
+        log("config: 512 batched instances ...")
+        others["batched_instances"] = config_batched(pa, ctx, synth)
+        log(f"  {others['batched_instances']}")
+        torch.cuda.empty_cache()
 
     cpu = None
     if not args.no_cpu_baseline:
