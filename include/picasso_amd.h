@@ -256,6 +256,17 @@ int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh,
                  uint32_t* keys, uint32_t* perm, uint32_t* bin_start,
                  uint32_t* active);
 
+/* Gather particle fields into bin order (no reference function; Picasso
+ * never reorders its particles): for each of nfields fields, dst(s, c) =
+ * src(perm[s], c) for s < np and c < ncomp[field].  src and dst must not
+ * overlap.  With perm from pic_bin_sort this stores the particles bin by bin,
+ * after which the brick kernels read them with unit-stride loads (a PIC code
+ * re-sorting its particle arrays); the sort's perm then becomes nearly the
+ * identity on later steps. */
+int pic_permute(pic_context* ctx, int dtype, int64_t np, const uint32_t* perm,
+                int nfields, const pic_pfield* src, const pic_pfield* dst,
+                const int* ncomp);
+
 /* Step graphs (no reference counterpart: Kokkos launches each kernel).
  * Between pic_graph_begin and pic_graph_end the context's calls are
  * captured into a HIP graph instead of running; pic_graph_launch replays

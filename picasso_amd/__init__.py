@@ -370,6 +370,25 @@ class BinSort:
                          self.geom.c())
 
 
+def permute(ctx, perm, pairs):
+    """Gather particle fields into bin order: for each (src, dst) tensor pair
+    ([np, q] each, same dtype), dst[s] = src[perm[s]] (pic_permute)."""
+    if not pairs:
+        return
+    dt = _dtype_code(pairs[0][0])
+    np_ = pairs[0][0].shape[0]
+    n = len(pairs)
+    srcs = (_abi.PField * n)(*[_pfield(a, a.shape[1], "src") for a, _ in pairs])
+    dsts = (_abi.PField * n)(*[_pfield(b, b.shape[1], "dst") for _, b in pairs])
+    nc = (ctypes.c_int * n)(*[a.shape[1] for a, _ in pairs])
+    for a, b in pairs:
+        if _dtype_code(a) != dt or _dtype_code(b) != dt or a.shape != b.shape:
+            raise ValueError("permute: pairs must share dtype and shape")
+    rc = lib.pic_permute(ctx._bind_stream(), dt, np_, ctypes.c_void_p(perm.data_ptr()), n,
+                         srcs, dsts, nc)
+    check(rc, "pic_permute")
+
+
 def bin_count(geom):
     return check(lib.pic_bin_count(ctypes.byref(geom.c())), "pic_bin_count")
 

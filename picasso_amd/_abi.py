@@ -87,6 +87,8 @@ PROTOTYPES = {
     "pic_apic_g2p_update_F": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(Mesh), _c_i64,
                                        _P(PField), _P(Grid), _P(Grid), _P(PField), _c_double,
                                        _P(PField), _P(PField), _P(PField), _P(Bins)]),
+    "pic_permute": (_c_int, [_vp, _c_int, _c_i64, _vp, _c_int, _P(PField), _P(PField),
+                             _P(_c_int)]),
     "pic_graph_begin": (_c_int, [_vp]),
     "pic_graph_end": (_c_int, [_vp, _P(_vp)]),
     "pic_graph_launch": (_c_int, [_vp, _vp]),

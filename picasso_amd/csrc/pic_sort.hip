@@ -372,3 +372,46 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
     }
     return launch_status();
 }
+
+// ---------------------------------------------------------------------------
+// Particle fields gathered into bin order: dst(s, c) = src(perm[s], c).
+namespace pic {
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_permute(int64_t np, const uint32_t* perm, PField<T> src,
+                                                 PField<T> dst, int ncomp)
+{
+    for (int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; s < np;
+         s += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t sb = src.base(int64_t(perm[s])), db = dst.base(s);
+        for (int c = 0; c < ncomp; ++c) dst.at(db, c) = src.at(sb, c);
+    }
+}
+
+}  // namespace pic
+
+extern "C" int pic_permute(pic_context* ctx, int dtype, int64_t np, const uint32_t* perm,
+                           int nfields, const pic_pfield* src, const pic_pfield* dst,
+                           const int* ncomp)
+{
+    if (!ctx || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 || nfields < 0 ||
+        (nfields > 0 && (!src || !dst || !ncomp)))
+        return PIC_ERR_ARG;
+    if (np == 0 || nfields == 0) return PIC_OK;
+    if (!perm) return PIC_ERR_ARG;
+    for (int f = 0; f < nfields; ++f)
+        if (!valid_pfield(&src[f]) || !valid_pfield(&dst[f]) || ncomp[f] < 1) return PIC_ERR_ARG;
+    DeviceGuard g(ctx->device);
+    if (!g.ok) return PIC_ERR_HIP;
+    TimedLaunch tl(ctx, "permute");
+    const unsigned b = (unsigned)std::min<int64_t>(blocks_for(np, 256), 65536);
+    for (int f = 0; f < nfields; ++f) {
+        if (dtype == PIC_F64)
+            k_permute<double><<<b, 256, 0, ctx->stream>>>(np, perm, make_pfield<double>(src[f]),
+                                                          make_pfield<double>(dst[f]), ncomp[f]);
+        else
+            k_permute<float><<<b, 256, 0, ctx->stream>>>(np, perm, make_pfield<float>(src[f]),
+                                                         make_pfield<float>(dst[f]), ncomp[f]);
+    }
+    return launch_status();
+}

@@ -645,3 +645,22 @@ def test_step_graph_replays_match_eager(pa, wl0, dtype):
             assert gu.rel_inf(gu.host(c), ref[2]) < tol * 10
         graph.close()
         ctx.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_permute_matches_gather(pa, dtype):
+    """pic_permute against torch indexing, SoA and AoS fields."""
+    import gpu_util as gu
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    rng = np.random.default_rng(11)
+    P = 5000
+    perm = torch.as_tensor(rng.permutation(P).astype(np.int32), device=gu.DEV)
+    a = gu.soa(rng.normal(size=(P, 3)), tdt)
+    b = gu.dev(rng.normal(size=(P, 12)), tdt)
+    a2 = torch.empty_like(a.t()).t()
+    b2 = torch.empty_like(b)
+    pa.permute(gu.ctx(), perm, [(a, a2), (b, b2)])
+    gu.ctx().synchronize()
+    idx = perm.long()
+    assert torch.equal(a2, a[idx])
+    assert torch.equal(b2, b[idx])
