@@ -4,8 +4,9 @@
 // for quadratic splines on nodes, mesh cells for linear/cubic), so every
 // particle of a bin shares one stencil and a brick's particles touch a fixed
 // (edge + order)^3 node footprint.  Workgroups walk the non-empty bricks in
-// list order, taking the next one from a counter (neighbouring bricks run at
-// the same time and share their halo nodes in L2; uneven bricks balance).
+// list order, each XCD over its own contiguous part of the list, so that
+// neighbouring bricks run at the same time on one XCD and share their halo
+// nodes in its L2.
 //
 // p2g (4^3-bin bricks): four work-items per bin, one per grid component
 //   (mass, momentum x/y/z).  Each walks every particle of its bin, evaluates
@@ -72,10 +73,11 @@ struct P2GShape {
                                                                               : 4;
     static constexpr int CPR = 4 / NROUND;  // components per round
     // Particles per staging chunk: FP32 fits a typical brick (8 ppc x 64
-    // bins) in one chunk at four workgroups per compute unit; FP64 is
+    // bins) in one chunk at four workgroups per compute unit (616 x 64 B of
+    // staging plus 1 KB of ranges stays under 40 KB); FP64 is
     // register-limited to three and runs faster with smaller chunks and more
     // resident workgroups.
-    static constexpr int CAP = sizeof(T) == 4 ? 624 : 200;
+    static constexpr int CAP = sizeof(T) == 4 ? 616 : 200;
     static constexpr int STAGE = CAP * 16;                   // scalars
     static constexpr int SUMS = CPR * SLOTS;
     static constexpr int LDS = STAGE > SUMS ? STAGE : SUMS;
@@ -137,20 +139,25 @@ __device__ __forceinline__ void p2g_component(const Spline<ORDER, T>& sd, T m, T
 }
 
 template <int ORDER, typename T>
-__device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b, T* lds,
-                                              uint32_t* rng)
+using P2GState = BrickStage<(P2GShape<ORDER, T>::CAP + P2GShape<ORDER, T>::NT - 1) /
+                                P2GShape<ORDER, T>::NT,
+                            P2GShape<ORDER, T>::NT, P2GShape<ORDER, T>::BINS>;
+
+template <int ORDER, typename T>
+__device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const BrickItem& it,
+                                              BrickWalk<true, BrickArgs<T>>& w, T* lds,
+                                              P2GState<ORDER, T>& st)
 {
     using S = P2GShape<ORDER, T>;
     using T4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
     constexpr int NK = S::NK, NN = S::NN, BINS = S::BINS;
     constexpr int NT = S::NT, CAP = S::CAP;
+    constexpr int K = (CAP + NT - 1) / NT;
     const ApicArgs<T>& a = g.a;
-    const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
+    const uint32_t beg = it.beg, end = it.end;
     int f0[3];
-    brick_origin<PB>(g, b, f0);
-
-    bin_ranges<BINS, NT>(g, b, beg, end, rng);
-    const uint32_t kb = b * BINS;
+    brick_origin<PB>(g, it.b, f0);
+    const uint32_t kb = it.b * BINS;
 
     const int l = int(threadIdx.x) >> 2, comp = int(threadIdx.x) & 3;
     const int grp = comp > 0 ? comp : 1;
@@ -159,15 +166,18 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
     T acc[NN];
 #pragma unroll
     for (int n = 0; n < NN; ++n) acc[n] = T(0);
-    const uint32_t b0 = rng[l], b1 = rng[BINS + l];
     for (uint32_t c0 = beg; c0 < end; c0 += CAP) {
         const uint32_t c1 = min(end, c0 + uint32_t(CAP));
-        // Stage the chunk: consecutive work-items load consecutive particles.
-        for (uint32_t s = c0 + threadIdx.x; s < c1; s += NT) {
-            const int64_t p = int64_t(a.perm[s]);
+        uint32_t* rng = st.begin_chunk(g, a.perm, beg, c0, c1);
+        // Stage the chunk: consecutive work-items, consecutive particles.
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t s = c0 + threadIdx.x + uint32_t(k * NT);
+            if (s >= c1) continue;
+            const int64_t p = int64_t(st.cs.perm[k]);
             const int64_t xb = a.x.base(p), cb = a.c.base(p);
             const int q = int(s - c0);
-            const int lb = int(g.keys[s] - kb);
+            const int lb = int(st.cs.key[k] - kb);
             const int eb[3] = {f0[0] + lb / (PB * PB), f0[1] + (lb / PB) % PB, f0[2] + lb % PB};
             T4 v;
             T* fr = &v.x;
@@ -195,14 +205,16 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
                 v.w = a.c.at(cb, 9 + d);
                 *reinterpret_cast<T4*>(lds + stage_at(q, 1 + d)) = v;
             }
+            st.cs.template note<BINS>(k, q, kb, rng);
         }
+        st.end_chunk();
         __syncthreads();
-        const uint32_t s1 = min(b1, c1);
-        for (uint32_t s = max(b0, c0); s < s1; ++s) {
-            const int q = int(s - c0);
-            const T4 fm = *reinterpret_cast<const T4*>(lds + stage_at(q, 0));
+        if (c0 == beg) w.fetch();
+        const uint32_t q0 = rng[l], q1 = rng[BINS + l];
+        for (uint32_t q = q0; q < q1; ++q) {
+            const T4 fm = *reinterpret_cast<const T4*>(lds + stage_at(int(q), 0));
             if (fm.x == T(STALE_F)) continue;
-            T4 ub = *reinterpret_cast<const T4*>(lds + stage_at(q, grp));
+            T4 ub = *reinterpret_cast<const T4*>(lds + stage_at(int(q), grp));
             if (comp == 0) {
                 ub.x = T(1);
                 ub.y = ub.z = ub.w = T(0);
@@ -222,9 +234,11 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, uint32_t b,
 #pragma unroll
             for (int n = 0; n < NN; ++n) dst[n * BINS * S::CPR] = acc[n];
         }
+        if (r == 0) w.publish();
         __syncthreads();
+        if (r == 0) st.prefetch_next(g, a.perm, w, CAP);
         gather_flush<NK, PB, NT, S::CPR>(lds, f0, r_lo, 1, a.gm, a.gmu, a.err);
-        __syncthreads();
+        if (r + 1 < S::NROUND) __syncthreads();
     }
 }
 
@@ -234,21 +248,27 @@ __global__ __launch_bounds__((P2GShape<ORDER, T>::NT), (P2GShape<ORDER, T>::WAVE
 {
     using S = P2GShape<ORDER, T>;
     __shared__ __align__(16) T lds[S::LDS];
-    __shared__ uint32_t rng[2 * S::BINS];
-    __shared__ uint32_t slot[2];
-    walk_bricks<true>(g, slot, [&](uint32_t b) { p2g_one_brick<ORDER, T>(g, b, lds, rng); });
+    __shared__ uint32_t rng[2 * 2 * S::BINS];
+    __shared__ BrickItem q[2];
+    for (int i = threadIdx.x; i < 4 * S::BINS; i += S::NT) rng[i] = 0u;
+    P2GState<ORDER, T> st{rng, 0, false, {}};
+    walk_bricks<true>(g, q, [&](const BrickItem& it, BrickWalk<true, BrickArgs<T>>& w) {
+        p2g_one_brick<ORDER, T>(g, it, w, lds, st);
+    });
 }
 
 template <int BB, int ORDER, typename T, bool GF>
-__device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, uint32_t b, T* us)
+__device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const BrickItem& it,
+                                              BrickWalk<false, BrickArgs<T>>& w, T* us)
 {
     constexpr int FN = BB + ORDER;
     constexpr int FV = FN * FN * FN;
-    const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
+    const uint32_t beg = it.beg, end = it.end;
     int f0[3];
-    brick_origin<BB>(g, b, f0);
+    brick_origin<BB>(g, it.b, f0);
     const ApicArgs<T>& a = g.a;
     const GView<T>& gu = a.gmu;
+    w.fetch();
     for (int q = threadIdx.x; q < FV; q += BT) {
         const int ni = f0[0] + q / (FN * FN), nj = f0[1] + (q / FN) % FN, nk = f0[2] + q % FN;
         const bool in = gu.covers(0, ni, ni) && gu.covers(1, nj, nj) && gu.covers(2, nk, nk);
@@ -259,6 +279,7 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, uint32_t b,
 #pragma unroll
         for (int d = 0; d < 3; ++d) us[d * FV + q] = u[d];
     }
+    w.publish();
     __syncthreads();
 
     // Up to R particles per work-item at a time, with all their position
@@ -304,7 +325,6 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, uint32_t b,
             if constexpr (GF) store_F_update(a, pp[r], L);
         }
     }
-    __syncthreads();
 }
 
 template <int BB, int ORDER, typename T, bool GF>
@@ -312,8 +332,10 @@ __global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
 {
     constexpr int FN = BB + ORDER;
     __shared__ T us[3 * FN * FN * FN];
-    __shared__ uint32_t slot[2];
-    walk_bricks<false>(g, slot, [&](uint32_t b) { g2p_one_brick<BB, ORDER, T, GF>(g, b, us); });
+    __shared__ BrickItem q[2];
+    walk_bricks<false>(g, q, [&](const BrickItem& it, BrickWalk<false, BrickArgs<T>>& w) {
+        g2p_one_brick<BB, ORDER, T, GF>(g, it, w, us);
+    });
 }
 
 template <typename T>

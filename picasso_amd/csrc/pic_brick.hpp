@@ -1,10 +1,9 @@
 // Brick machinery shared by the APIC and PolyPIC brick kernels: the walk
-// over the non-empty bricks of a pic_bin_sort binning, bin ranges, the
-// stale-particle list, the staged-particle layout and the node gather that
-// turns per-bin stencil sums into one atomic per footprint node.
+// over the non-empty bricks of a pic_bin_sort binning, chunk-local bin
+// ranges, the stale-particle list, the staged-particle layout and the node
+// gather that turns per-bin stencil sums into one atomic per footprint node.
 //
-// Device helpers take any argument struct G with the fields of BrickWalk
-// (keys, bin_start, active, walk, stale, nbricks, cell_lo, nb, knot_off).
+// Device helpers take any argument struct G carrying PIC_BRICK_WALK_FIELDS.
 #pragma once
 
 #include "pic_common.hpp"
@@ -12,17 +11,33 @@
 
 namespace pic {
 
+// One brick of the walk: walk index t, brick id b and its slot range
+// [beg, end) in the sorted order.
+struct alignas(16) BrickItem {
+    uint32_t t, b, beg, end;
+};
+
 // Binning fields every brick kernel argument struct carries.
 #define PIC_BRICK_WALK_FIELDS                                                        \
     const uint32_t* keys;                                                            \
     const uint32_t* bin_start;                                                       \
     const uint32_t* active; /* [count, ids...] of non-empty bricks, or null */       \
-    uint32_t* walk;         /* brick-walk counter, zero at launch */                 \
+    const BrickItem* items; /* one per walk index, written by k_walk_items */        \
+    uint32_t* walk;         /* WALK_LANES counters, WALK_PAD words apart, zero */    \
     uint32_t* stale;        /* [count, slots...] of particles for the slow path */   \
     uint32_t nbricks;                                                                \
     int32_t cell_lo[3];                                                              \
     int32_t nb[3];                                                                   \
     int32_t knot_off; /* first knot index = bin index + knot_off */
+
+// The dynamic walk draws walk indices from WALK_LANES counters: lane x hands
+// out the indices congruent to x modulo WALK_LANES, in order.  Every draw on
+// one address is serialised by the memory system, so one counter shared by
+// a thousand workgroups caps the brick rate; eight counters on separate
+// lines keep the rate up while all workgroups still sweep one compact front
+// of the list.
+constexpr int WALK_LANES = 8;
+constexpr int WALK_PAD = 64;  // words between counters (256 B)
 
 template <int BB, typename G>
 __device__ __forceinline__ void brick_origin(const G& g, uint32_t b, int f0[3])
@@ -43,64 +58,187 @@ __device__ __forceinline__ uint32_t walk_count(const G& g)
     return g.active ? g.active[0] : g.nbricks;
 }
 
+// Item of walk index t (past the end: t with an empty range).
 template <typename G>
-__device__ __forceinline__ uint32_t walk_brick(const G& g, uint32_t t)
+__device__ __forceinline__ BrickItem load_item(const G& g, uint32_t t, uint32_t nw)
 {
-    return g.active ? g.active[1 + t] : t;
+    if (t < nw) return g.items[t];
+    return BrickItem{t, 0u, 0u, 0u};
 }
 
-// Drive body(brick) over the walk.  A workgroup's first index is its
-// blockIdx.  With DYN, each later one is requested from the counter when the
-// previous brick starts (the request's latency hides behind that brick), so
-// bricks start in list order whatever each one costs; every workgroup ends
-// after drawing an index past the end.  `slot` is a two-word LDS buffer
-// (alternating, so no extra barrier is needed when a brick is skipped).
-// Without DYN the walk is a plain grid stride, which suits kernels whose
-// bricks cost about the same (a contended counter then only adds latency).
-template <bool DYN, typename G, typename Body>
-__device__ __forceinline__ void walk_bricks(const G& g, uint32_t* slot, Body&& body)
+// The items of the walk in one flat array, so that a workgroup resolves its
+// next brick with one load instead of the chain active -> bin_start.  Also
+// clears the stale count.
+template <int = 0>
+__global__ __launch_bounds__(256) void k_walk_items(const uint32_t* active,
+                                                    const uint32_t* bin_start, uint32_t nbricks,
+                                                    BrickItem* items, uint32_t* stale)
 {
-    const uint32_t nw = walk_count(g);
-    uint32_t t = blockIdx.x;
-    for (int it = 0; t < nw; ++it) {
-        uint32_t nxt = t + gridDim.x;
-        if (DYN && threadIdx.x == 0) nxt = gridDim.x + atomicAdd(g.walk, 1u);
-        const uint32_t b = walk_brick(g, t);
-        if (g.bin_start[b] != g.bin_start[b + 1]) body(b);
-        if constexpr (DYN) {
-            if (threadIdx.x == 0) slot[it & 1] = nxt;
-            __syncthreads();
-            t = slot[it & 1];
-        } else {
-            __syncthreads();
-            t = nxt;
-        }
+    if (blockIdx.x == 0 && threadIdx.x == 0) stale[0] = 0u;
+    const uint32_t nw = active ? active[0] : nbricks;
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nw; t += gridDim.x * blockDim.x) {
+        const uint32_t b = active ? active[1 + t] : t;
+        items[t] = BrickItem{t, b, bin_start[b], bin_start[b + 1]};
     }
 }
+
+// Walk state of a workgroup.  Brick n's item sits in q[n & 1] (LDS).
+// Work-item 0 draws the walk index of brick n+1 at brick n's start
+// (draw), loads its item once the draw has returned (fetch) and stores it
+// into q[(n+1) & 1] (publish); after the next barrier every work-item can
+// read it (peek), e.g. to prefetch the next brick's slots.  The bodies call
+// fetch() and publish() once per brick, in that order, at points where
+// their latency hides behind the brick's own loads.
+template <bool DYN, typename G>
+struct BrickWalk {
+    const G& g;
+    BrickItem* q;
+    uint32_t nw;
+    uint32_t lane, nl;  // counter lane of this workgroup, lanes in use
+    int n;
+    uint32_t t_next;  // work-item 0
+    BrickItem nxt;    // work-item 0
+
+    __device__ __forceinline__ void draw(uint32_t t)
+    {
+        if (threadIdx.x != 0) return;
+        if constexpr (DYN)
+            t_next = lane + nl * (gridDim.x / nl + atomicAdd(g.walk + lane * WALK_PAD, 1u));
+        else
+            t_next = t + gridDim.x;
+    }
+    __device__ __forceinline__ void fetch()
+    {
+        if (threadIdx.x == 0) nxt = load_item(g, t_next, nw);
+    }
+    __device__ __forceinline__ void publish()
+    {
+        if (threadIdx.x == 0) q[(n + 1) & 1] = nxt;
+    }
+    __device__ __forceinline__ BrickItem peek() const { return q[(n + 1) & 1]; }
+    __device__ __forceinline__ bool has_work(const BrickItem& it) const
+    {
+        return it.t < nw && it.beg < it.end;
+    }
+};
+
+// Drive body(item, walk) over the walk.  Without DYN the walk is a plain
+// grid stride (workgroup b takes b, b + grid, ...), which suits kernels
+// without atomics whose bricks cost about the same.  With DYN, when the grid
+// is a multiple of WALK_LANES, workgroup b (b = 8m + r) starts at walk index
+// 8m + lane with lane = (r + m) mod 8 and then draws from counter `lane`;
+// so each lane's workgroups sit on every XCD (blockIdx modulo 8 picks the
+// XCD), the lanes advance at the same pace and bricks start in list order
+// whatever each one costs: neighbouring bricks run at the same time and
+// share their halo nodes in L2.  Otherwise one counter serves the grid.
+// Every workgroup ends after drawing an index past the end.  q is a
+// two-item LDS buffer.  Bodies do not end with a barrier: the walk places
+// one after each brick.
+template <bool DYN, typename G, typename Body>
+__device__ __forceinline__ void walk_bricks(const G& g, BrickItem* q, Body&& body)
+{
+    uint32_t nl = 1, lane = 0, t0 = blockIdx.x;
+    if (DYN && gridDim.x % WALK_LANES == 0) {
+        const uint32_t m = blockIdx.x / WALK_LANES;
+        nl = WALK_LANES;
+        lane = (blockIdx.x + m) % WALK_LANES;
+        t0 = lane + WALK_LANES * m;
+    }
+    BrickWalk<DYN, G> w{g, q, walk_count(g), lane, nl, 0, 0u, BrickItem{}};
+    if (threadIdx.x == 0) q[0] = load_item(g, t0, w.nw);
+    __syncthreads();
+    for (;; ++w.n) {
+        const BrickItem cur = q[w.n & 1];
+        if (cur.t >= w.nw) break;
+        w.draw(cur.t);
+        if (cur.beg < cur.end) {
+            body(cur, w);
+        } else {
+            w.fetch();
+            w.publish();
+        }
+        __syncthreads();
+    }
+}
+
+// Sorted slots of one staging chunk [c0, c1), K per work-item (slot
+// c0 + threadIdx.x + k NT): the particle index and the bin keys of the slot
+// and its neighbours inside the chunk (~0 past either end), from which the
+// chunk-local bin ranges follow.  Loaded either while the previous brick
+// is flushed or at staging time.
+template <int K, int NT>
+struct ChunkSlots {
+    uint32_t perm[K], key[K], prv[K], nxt[K];
+
+    template <typename G>
+    __device__ __forceinline__ void load(const G& g, const uint32_t* perm_v, uint32_t c0,
+                                         uint32_t c1)
+    {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t s = c0 + threadIdx.x + uint32_t(k * NT);
+            if (s < c1) {
+                perm[k] = perm_v[s];
+                key[k] = g.keys[s];
+                prv[k] = s > c0 ? g.keys[s - 1] : ~0u;
+                nxt[k] = s + 1 < c1 ? g.keys[s + 1] : ~0u;
+            }
+        }
+    }
+    // Chunk-local range [rng[bin], rng[BINS + bin]) of slot k's bin (chunk
+    // position q), from the neighbour keys.
+    template <int BINS>
+    __device__ __forceinline__ void note(int k, int q, uint32_t kb, uint32_t* rng) const
+    {
+        if (prv[k] != key[k]) rng[key[k] - kb] = uint32_t(q);
+        if (nxt[k] != key[k]) rng[BINS + key[k] - kb] = uint32_t(q + 1);
+    }
+};
+
+// Per-workgroup staging state of a p2g brick walk: double-buffered
+// chunk-local bin ranges (the buffer of the next chunk is cleared while the
+// current one is staged) and the slots of the next brick's first chunk,
+// prefetched while the current brick is flushed.
+template <int K, int NT, int BINS>
+struct BrickStage {
+    static_assert(2 * BINS <= NT, "range buffers are cleared one word per work-item");
+    uint32_t* rng;  // LDS [2][2 * BINS], zero at the start of the walk
+    int par;        // range buffer of the next chunk
+    bool have;      // cs holds the first chunk of the current brick
+    ChunkSlots<K, NT> cs;
+
+    // Slots of chunk [c0, c1) of a brick starting at beg; returns the range
+    // buffer the chunk's notes go to.
+    template <typename G>
+    __device__ __forceinline__ uint32_t* begin_chunk(const G& g, const uint32_t* perm_v,
+                                                     uint32_t beg, uint32_t c0, uint32_t c1)
+    {
+        if (!(have && c0 == beg)) cs.load(g, perm_v, c0, c1);
+        return rng + par * (2 * BINS);
+    }
+    // Before the barrier that ends the staging of a chunk.
+    __device__ __forceinline__ void end_chunk()
+    {
+        if (threadIdx.x < 2 * BINS) rng[(par ^ 1) * (2 * BINS) + threadIdx.x] = 0u;
+        par ^= 1;
+    }
+    // After the barrier that follows the walk's publish(): prefetch the
+    // next brick's first chunk.
+    template <typename G, typename W>
+    __device__ __forceinline__ void prefetch_next(const G& g, const uint32_t* perm_v, const W& w,
+                                                  uint32_t cap)
+    {
+        const BrickItem nx = w.peek();
+        have = w.has_work(nx);
+        if (have) cs.load(g, perm_v, nx.beg, min(nx.end, nx.beg + cap));
+    }
+};
 
 template <typename G>
 __device__ __forceinline__ void push_stale(const G& g, uint32_t s)
 {
     const uint32_t at = atomicAdd(g.stale, 1u);
     g.stale[1 + at] = s;
-}
-
-// Bin ranges [rng[l], rng[BINS + l]) of brick b (slots [beg, end)) from the
-// sorted keys, by neighbour comparison (empty bins: 0, 0).  Ends with a
-// barrier.
-template <int BINS, int NT, typename G>
-__device__ __forceinline__ void bin_ranges(const G& g, uint32_t b, uint32_t beg, uint32_t end,
-                                           uint32_t* rng)
-{
-    for (int l = threadIdx.x; l < 2 * BINS; l += NT) rng[l] = 0u;
-    __syncthreads();
-    const uint32_t kb = b * BINS;
-    for (uint32_t s = beg + threadIdx.x; s < end; s += NT) {
-        const uint32_t key = g.keys[s];
-        if (s == beg || g.keys[s - 1] != key) rng[key - kb] = s;
-        if (s + 1 == end || g.keys[s + 1] != key) rng[BINS + key - kb] = s + 1;
-    }
-    __syncthreads();
 }
 
 constexpr double STALE_F = 4.0;  // staged f0 of a particle the brick skips
@@ -209,12 +347,21 @@ inline bool setup_walk(pic_context* ctx, int order, int entity, int64_t np, cons
         g.cell_lo[d] = bins.geom.cell_lo[d];
         g.nb[d] = (bins.geom.ncell[d] + bb - 1) / bb;
     }
-    // scratch: walk counter | stale count | stale slots[np]
-    uint32_t* w = static_cast<uint32_t*>(scratch(ctx, size_t(np + 2) * sizeof(uint32_t)));
-    if (!w) return false;
+    // scratch: items[nbricks] | walk counters | stale count | stale slots[np]
+    const size_t ibytes = size_t(nb) * sizeof(BrickItem);
+    const size_t words = size_t(WALK_LANES) * WALK_PAD + 1 + size_t(np);
+    char* base = static_cast<char*>(scratch(ctx, ibytes + words * sizeof(uint32_t)));
+    if (!base) return false;
+    BrickItem* items = reinterpret_cast<BrickItem*>(base);
+    uint32_t* w = reinterpret_cast<uint32_t*>(base + ibytes);
+    g.items = items;
     g.walk = w;
-    g.stale = w + 1;
-    return fill_u32(ctx, w, 2, 0u) == PIC_OK;
+    g.stale = w + WALK_LANES * WALK_PAD;
+    if (fill_u32(ctx, w, WALK_LANES * WALK_PAD, 0u) != PIC_OK) return false;
+    const unsigned nblk = unsigned(std::min<int64_t>((nb + 255) / 256, 2048));
+    k_walk_items<<<nblk, 256, 0, ctx->stream>>>(bins.active, bins.bin_start, uint32_t(nb), items,
+                                                  g.stale);
+    return launch_status() == 0;
 }
 
 }  // namespace pic

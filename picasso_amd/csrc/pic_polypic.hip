@@ -267,8 +267,16 @@ __device__ __forceinline__ T nested_horner(const T (&C)[E][E][E], const T y[3])
 }
 
 template <int ORDER, typename T>
-__device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g, uint32_t b, T* lds,
-                                                      uint32_t* pidx, uint32_t* rng)
+using PolyStage = BrickStage<(PolyBrickShape<ORDER, T>::CAP + PolyBrickShape<ORDER, T>::NT - 1) /
+                                 PolyBrickShape<ORDER, T>::NT,
+                             PolyBrickShape<ORDER, T>::NT, PolyBrickShape<ORDER, T>::BINS>;
+
+template <int ORDER, typename T>
+__device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
+                                                      const BrickItem& it,
+                                                      BrickWalk<true, PolyBrickArgs<T>>& w,
+                                                      T* lds, uint32_t* pidx,
+                                                      PolyStage<ORDER, T>& st)
 {
     using S = PolyBrickShape<ORDER, T>;
     using T4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
@@ -276,12 +284,12 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
     constexpr int NM = Modes<ORDER>::NM;
     constexpr int E = ORDER == 1 ? 2 : 3;
     constexpr int FIRST = Knots<ORDER>::FIRST;
+    constexpr int K = (CAP + NT - 1) / NT;
     const PolyArgs<T>& a = g.a;
-    const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
+    const uint32_t beg = it.beg, end = it.end;
     int f0[3];
-    brick_origin<PB>(g, b, f0);
-    bin_ranges<BINS, NT>(g, b, beg, end, rng);
-    const uint32_t kb = b * BINS;
+    brick_origin<PB>(g, it.b, f0);
+    const uint32_t kb = it.b * BINS;
 
     const int l = int(threadIdx.x) >> 2, comp = int(threadIdx.x) & 3;
     const int d = comp > 0 ? comp - 1 : 0;
@@ -289,13 +297,16 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
     T acc[NN];
 #pragma unroll
     for (int n = 0; n < NN; ++n) acc[n] = T(0);
-    const uint32_t b0 = rng[l], b1 = rng[BINS + l];
     for (uint32_t c0 = beg; c0 < end; c0 += CAP) {
         const uint32_t c1 = min(end, c0 + uint32_t(CAP));
-        for (uint32_t s = c0 + threadIdx.x; s < c1; s += NT) {
-            const int64_t p = int64_t(a.perm[s]);
+        uint32_t* rng = st.begin_chunk(g, a.perm, beg, c0, c1);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t s = c0 + threadIdx.x + uint32_t(k * NT);
+            if (s >= c1) continue;
+            const int64_t p = int64_t(st.cs.perm[k]);
             const int q = int(s - c0);
-            const int lb = int(g.keys[s] - kb);
+            const int lb = int(st.cs.key[k] - kb);
             const int eb[3] = {f0[0] + lb / (PB * PB), f0[1] + (lb / PB) % PB, f0[2] + lb % PB};
             const int64_t xb = a.x.base(p);
             T4 v;
@@ -333,11 +344,14 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
                 *reinterpret_cast<T4*>(lds + stage_at(q, 1 + r)) = v;
             }
             pidx[q] = uint32_t(p);
+            st.cs.template note<BINS>(k, q, kb, rng);
         }
+        st.end_chunk();
         __syncthreads();
-        const uint32_t s1 = min(b1, c1);
-        for (uint32_t s = max(b0, c0); s < s1; ++s) {
-            const int q = int(s - c0);
+        if (c0 == beg) w.fetch();
+        const uint32_t q0 = rng[l], q1 = rng[BINS + l];
+        for (uint32_t qq = q0; qq < q1; ++qq) {
+            const int q = int(qq);
             const T4 fm = *reinterpret_cast<const T4*>(lds + stage_at(q, 0));
             if (fm.x == T(STALE_F)) continue;
             T ai[3][3];
@@ -402,9 +416,11 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
 #pragma unroll
             for (int n = 0; n < NN; ++n) dst[n * BINS * S::CPR] = acc[n];
         }
+        if (r == 0) w.publish();
         __syncthreads();
+        if (r == 0) st.prefetch_next(g, a.perm, w, CAP);
         gather_flush<NK, PB, NT, S::CPR>(lds, f0, r_lo, 1, a.gm, a.gmu, a.err);
-        __syncthreads();
+        if (r + 1 < S::NROUND) __syncthreads();
     }
 }
 
@@ -414,7 +430,9 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
 // (F_e = weight, gradient, half second derivative), summed one dimension
 // at a time (k, then j, then i) instead of per node and mode.
 template <int ORDER, typename T>
-__device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g, uint32_t b,
+__device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
+                                                      const BrickItem& it,
+                                                      BrickWalk<false, PolyBrickArgs<T>>& w,
                                                       T* us)
 {
     constexpr int PB = 4, NT = 256;
@@ -424,10 +442,11 @@ __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
     constexpr int NM = Modes<ORDER>::NM;
     constexpr int E = ORDER == 1 ? 2 : 3;
     const PolyArgs<T>& a = g.a;
-    const uint32_t beg = g.bin_start[b], end = g.bin_start[b + 1];
+    const uint32_t beg = it.beg, end = it.end;
     int f0[3];
-    brick_origin<PB>(g, b, f0);
+    brick_origin<PB>(g, it.b, f0);
     const GView<T>& gu = a.gmu;
+    w.fetch();
     for (int q = threadIdx.x; q < FV; q += NT) {
         const int ni = f0[0] + q / (FN * FN), nj = f0[1] + (q / FN) % FN, nk = f0[2] + q % FN;
         const bool in = gu.covers(0, ni, ni) && gu.covers(1, nj, nj) && gu.covers(2, nk, nk);
@@ -435,6 +454,7 @@ __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
 #pragma unroll
         for (int d = 0; d < 3; ++d) us[d * FV + q] = in ? gu.data[off + d * gu.cs] : T(0);
     }
+    w.publish();
     __syncthreads();
     for (uint32_t s = beg + threadIdx.x; s < end; s += NT) {
         const int64_t p = int64_t(a.perm[s]);
@@ -515,7 +535,6 @@ __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
             for (int d = 0; d < 3; ++d)
                 a.c.at(cb, r * 3 + d) = cm[MODE_EXP[r][0]][MODE_EXP[r][1]][MODE_EXP[r][2]][d];
     }
-    __syncthreads();
 }
 
 template <int ORDER, typename T>
@@ -523,8 +542,10 @@ __global__ __launch_bounds__(256) void k_polypic_g2p_brick(PolyBrickArgs<T> g)
 {
     constexpr int FN = 4 + ORDER;
     __shared__ T us[3 * FN * FN * FN];
-    __shared__ uint32_t slot[2];
-    walk_bricks<false>(g, slot, [&](uint32_t b) { polypic_g2p_one_brick<ORDER, T>(g, b, us); });
+    __shared__ BrickItem q[2];
+    walk_bricks<false>(g, q, [&](const BrickItem& it, BrickWalk<false, PolyBrickArgs<T>>& w) {
+        polypic_g2p_one_brick<ORDER, T>(g, it, w, us);
+    });
 }
 
 template <int ORDER, typename T>
@@ -534,10 +555,12 @@ __global__ __launch_bounds__((PolyBrickShape<ORDER, T>::NT)) void k_polypic_p2g_
     using S = PolyBrickShape<ORDER, T>;
     __shared__ __align__(16) T lds[S::LDS];
     __shared__ uint32_t pidx[S::CAP];
-    __shared__ uint32_t rng[2 * S::BINS];
-    __shared__ uint32_t slot[2];
-    walk_bricks<true>(g, slot, [&](uint32_t b) {
-        polypic_p2g_one_brick<ORDER, T>(g, b, lds, pidx, rng);
+    __shared__ uint32_t rng[2 * 2 * S::BINS];
+    __shared__ BrickItem q[2];
+    for (int i = threadIdx.x; i < 4 * S::BINS; i += S::NT) rng[i] = 0u;
+    PolyStage<ORDER, T> st{rng, 0, false, {}};
+    walk_bricks<true>(g, q, [&](const BrickItem& it, BrickWalk<true, PolyBrickArgs<T>>& w) {
+        polypic_p2g_one_brick<ORDER, T>(g, it, w, lds, pidx, st);
     });
 }
 
