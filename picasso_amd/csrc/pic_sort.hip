@@ -6,7 +6,8 @@
 //
 // Passes
 //  (1) keys in input order + per-brick counts (wave-aggregated atomics);
-//  (2) exclusive scan of the counts into bin_start;
+//  (2) exclusive scan of the counts into bin_start, and the list of
+//      non-empty bricks, in two passes over tiles of counts;
 //  (3) scatter of particle indices into their brick segments (wave-aggregated
 //      cursors: order inside a segment depends on scheduling);
 //  (4) one workgroup per brick: LDS counting sort by bin, then each bin's
@@ -53,42 +54,96 @@ __device__ __forceinline__ uint32_t wave_agg_inc(uint32_t* base, uint32_t key, b
     return slot;
 }
 
+// Two-phase form of wave_agg_inc for several calls in flight: issue() does
+// the aggregated atomics and keeps, per lane, its group's leader and its
+// rank; resolve() reads the leader's returned base.  The atomics of all
+// issued calls are then outstanding together instead of one at a time.
+struct AggTicket {
+    uint32_t base;  // returned old value (meaningful in the leader lane)
+    int leader;
+    uint32_t rank;
+};
+
+__device__ __forceinline__ AggTicket wave_agg_issue(uint32_t* base, uint32_t key, bool active)
+{
+    const int lane = __lane_id();
+    uint64_t pending = __ballot(active);
+    AggTicket t{0u, 0, 0u};
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const uint32_t lkey = __shfl(key, leader);
+        const uint64_t same = __ballot(active && key == lkey) & pending;
+        if (lane == leader) t.base = atomicAdd(base + lkey, (uint32_t)__popcll(same));
+        if ((same >> lane) & 1ull) {
+            t.leader = leader;
+            t.rank = (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        }
+        pending &= ~same;
+    }
+    return t;
+}
+
+__device__ __forceinline__ uint32_t wave_agg_resolve(const AggTicket& t)
+{
+    return uint32_t(__shfl((int)t.base, t.leader)) + t.rank;
+}
+
+// Particles per work-item in the per-particle sort passes: their loads are
+// all in flight together, and the grid is a quarter of the particles.
+constexpr int SORT_ITEMS = 4;
+constexpr int SORT_PT = 256;  // work-items per block of those passes
+
 // Particles outside the bins are counted in count[nbricks] (they are placed
 // after every brick, so perm stays a permutation even then).
 template <typename T>
-__global__ __launch_bounds__(256) void k_cell_keys(int64_t np, PField<T> x, KeyGeom g,
-                                                   uint32_t nbricks, uint32_t* keys,
-                                                   uint32_t* count, int* err)
+__global__ __launch_bounds__(SORT_PT) void k_cell_keys(int64_t np, PField<T> x, KeyGeom g,
+                                                       uint32_t nbricks, uint32_t* keys,
+                                                       uint32_t* count, int* err)
 {
-    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool live = p < np;
-    uint32_t key = 0xffffffffu, b = 0;
-    if (live) {
-        key = cell_key(g, x, p, &b);
-        keys[p] = key;
-        if (key == 0xffffffffu) {
-            atomicOr(err, ERRBIT_CELL);
-            b = nbricks;
-        }
+    const int64_t p0 = int64_t(blockIdx.x) * (SORT_PT * SORT_ITEMS) + threadIdx.x;
+    uint32_t key[SORT_ITEMS], b[SORT_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+        const int64_t p = p0 + k * SORT_PT;
+        key[k] = 0xffffffffu;
+        b[k] = 0;
+        if (p < np) key[k] = cell_key(g, x, p, &b[k]);
     }
-    (void)wave_agg_inc(count, b, live);
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+        const int64_t p = p0 + k * SORT_PT;
+        const bool live = p < np;
+        if (live) {
+            keys[p] = key[k];
+            if (key[k] == 0xffffffffu) {
+                atomicOr(err, ERRBIT_CELL);
+                b[k] = nbricks;
+            }
+        }
+        (void)wave_agg_inc(count, b[k], live);
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Device-wide exclusive scan of uint32.
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total)
+// bin_start (exclusive scan of the nb + 1 counts) and the list of non-empty
+// bricks, in two passes over tiles of SCAN_TILE counts: per-tile sums of the
+// counts and of the non-empty flags, then per tile the sums of the tiles
+// before it and a block scan of its own values.  A count and a flag travel
+// as one 64-bit value (flag count high, particle count low: a particle
+// total below 2^32 never carries into the high word).
+__device__ __forceinline__ uint64_t block_exclusive_scan64(uint64_t v, uint64_t* total)
 {
-    __shared__ uint32_t wsum[SCAN_THREADS / 64];
+    __shared__ uint64_t wsum[SCAN_THREADS / 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t inc = v;
+    uint64_t inc = v;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(inc, o);
+        const uint64_t y = __shfl_up(inc, o);
         if (lane >= o) inc += y;
     }
     if (lane == 63) wsum[wave] = inc;
     __syncthreads();
-    uint32_t before = 0, all = 0;
+    uint64_t before = 0, all = 0;
 #pragma unroll
     for (int w = 0; w < SCAN_THREADS / 64; ++w) {
         before += (w < wave) ? wsum[w] : 0u;
@@ -99,79 +154,128 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
     return before + inc - v;
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_tile(const uint32_t* in, uint32_t* out,
-                                                            int64_t n, uint32_t* sums)
+// Count and non-empty flag of entry i of the nb + 1 counts (entry nb, the
+// particles outside the bins, is no brick).
+__device__ __forceinline__ uint64_t count_pair(const uint32_t* count, int64_t i, int64_t nb)
+{
+    if (i > nb) return 0u;
+    const uint32_t c = count[i];
+    return uint64_t(c) | (uint64_t(i < nb && c > 0u ? 1u : 0u) << 32);
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_tile_sums(const uint32_t* count, int64_t nb,
+                                                            uint64_t* sums)
 {
     const int64_t base = int64_t(blockIdx.x) * SCAN_TILE + int64_t(threadIdx.x) * SCAN_ITEMS;
-    uint32_t v[SCAN_ITEMS];
-    uint32_t s = 0;
+    uint64_t s = 0;
+#pragma unroll
+    for (int q = 0; q < SCAN_ITEMS; ++q) s += count_pair(count, base + q, nb);
+    uint64_t total;
+    (void)block_exclusive_scan64(s, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// In-place exclusive scan of the tile sums by one workgroup, for grids of
+// more than SCAN_TILE tiles (fewer are summed by each tile directly).
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_sums(uint64_t* sums, int64_t n)
+{
+    uint64_t carry = 0;
+    for (int64_t c0 = 0; c0 < n; c0 += SCAN_TILE) {
+        const int64_t base = c0 + int64_t(threadIdx.x) * SCAN_ITEMS;
+        uint64_t v[SCAN_ITEMS];
+        uint64_t s = 0;
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS; ++q) {
+            v[q] = base + q < n ? sums[base + q] : 0u;
+            s += v[q];
+        }
+        uint64_t total;
+        uint64_t run = carry + block_exclusive_scan64(s, &total);
+#pragma unroll
+        for (int q = 0; q < SCAN_ITEMS; ++q) {
+            if (base + q < n) sums[base + q] = run;
+            run += v[q];
+        }
+        carry += total;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_compact(const uint32_t* count, int64_t nb,
+                                                               const uint64_t* sums,
+                                                               bool prefixed, uint32_t* bin_start,
+                                                               uint32_t* active)
+{
+    // offset of this tile: the sum of the tiles before it
+    uint64_t tile_off;
+    if (prefixed) {
+        tile_off = sums[blockIdx.x];
+    } else {
+        uint64_t off = 0;
+        for (uint32_t u = threadIdx.x; u < blockIdx.x; u += SCAN_THREADS) off += sums[u];
+        (void)block_exclusive_scan64(off, &tile_off);
+    }
+    const int64_t base = int64_t(blockIdx.x) * SCAN_TILE + int64_t(threadIdx.x) * SCAN_ITEMS;
+    uint64_t v[SCAN_ITEMS];
+    uint64_t s = 0;
 #pragma unroll
     for (int q = 0; q < SCAN_ITEMS; ++q) {
-        v[q] = (base + q < n) ? in[base + q] : 0u;
+        v[q] = count_pair(count, base + q, nb);
         s += v[q];
     }
-    uint32_t total;
-    uint32_t run = block_exclusive_scan(s, &total);
+    uint64_t total;
+    uint64_t run = tile_off + block_exclusive_scan64(s, &total);
 #pragma unroll
     for (int q = 0; q < SCAN_ITEMS; ++q) {
-        if (base + q < n) out[base + q] = run;
+        const int64_t i = base + q;
+        if (i <= nb) {
+            bin_start[i] = uint32_t(run);
+            if (v[q] >> 32) active[1 + uint32_t(run >> 32)] = uint32_t(i);
+        }
         run += v[q];
     }
-    if (threadIdx.x == 0 && sums) sums[blockIdx.x] = total;
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1)
+        active[0] = uint32_t(run >> 32);
 }
 
-__global__ __launch_bounds__(SCAN_THREADS) void k_scan_add(uint32_t* out, int64_t n,
-                                                           const uint32_t* offs)
-{
-    const int64_t base = int64_t(blockIdx.x) * SCAN_TILE;
-    const uint32_t add = offs[blockIdx.x];
-    for (int q = threadIdx.x; q < SCAN_TILE; q += SCAN_THREADS)
-        if (base + q < n) out[base + q] += add;
-}
-
-// Words of temporary storage exclusive_scan needs for n values.
-int64_t scan_tmp_words(int64_t n)
-{
-    int64_t w = 0;
-    while (n > SCAN_TILE) {
-        n = (n + SCAN_TILE - 1) / SCAN_TILE;
-        w += n + 1;
-    }
-    return w + 1;
-}
-
-void exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp)
-{
-    const int64_t nt = (n + SCAN_TILE - 1) / SCAN_TILE;
-    if (nt <= 1) {
-        k_scan_tile<<<1, SCAN_THREADS, 0, st>>>(in, out, n, nullptr);
-        return;
-    }
-    uint32_t* sums = tmp;
-    k_scan_tile<<<(unsigned)nt, SCAN_THREADS, 0, st>>>(in, out, n, sums);
-    exclusive_scan(st, sums, sums, nt, tmp + nt + 1);
-    k_scan_add<<<(unsigned)nt, SCAN_THREADS, 0, st>>>(out, n, sums);
-}
 
 // ---------------------------------------------------------------------------
 // Particles outside the bins go straight to their final slots after the
 // last brick (key 0xffffffff, order unspecified: the call reports an error).
-__global__ __launch_bounds__(256) void k_bin_scatter(int64_t np, const uint32_t* keys,
-                                                     uint32_t bvol, uint32_t nbricks,
-                                                     const uint32_t* start, uint32_t* cursor,
-                                                     uint32_t* tmp, uint32_t* perm,
-                                                     uint32_t* keys_out)
+__global__ __launch_bounds__(SORT_PT) void k_bin_scatter(int64_t np, const uint32_t* keys,
+                                                         int32_t vshift, uint32_t bvol,
+                                                         uint32_t nbricks, const uint32_t* start,
+                                                         uint32_t* cursor, uint32_t* tmp,
+                                                         uint32_t* perm, uint32_t* keys_out)
 {
-    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    const bool live = p < np;
-    const uint32_t key = live ? keys[p] : 0xffffffffu;
-    const bool ok = key != 0xffffffffu;
-    const uint32_t b = ok ? key / bvol : nbricks;
-    const uint32_t r = wave_agg_inc(cursor, b, live);
-    if (live && ok) tmp[start[b] + r] = uint32_t(p);
-    if (live && !ok) {
-        perm[start[nbricks] + r] = uint32_t(p);
-        keys_out[start[nbricks] + r] = 0xffffffffu;
+    const int64_t p0 = int64_t(blockIdx.x) * (SORT_PT * SORT_ITEMS) + threadIdx.x;
+    uint32_t key[SORT_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+        const int64_t p = p0 + k * SORT_PT;
+        key[k] = p < np ? keys[p] : 0xffffffffu;
+    }
+    AggTicket tk[SORT_ITEMS];
+    uint32_t at[SORT_ITEMS];
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+        const bool live = p0 + k * SORT_PT < np;
+        const uint32_t b = key[k] == 0xffffffffu ? nbricks
+                           : vshift >= 0         ? key[k] >> vshift
+                                                 : key[k] / bvol;
+        at[k] = live ? start[b] : 0u;
+        tk[k] = wave_agg_issue(cursor, b, live);
+    }
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+        const int64_t p = p0 + k * SORT_PT;
+        const uint32_t r = wave_agg_resolve(tk[k]);
+        if (p >= np) continue;
+        if (key[k] != 0xffffffffu) {
+            tmp[at[k] + r] = uint32_t(p);
+        } else {
+            perm[at[k] + r] = uint32_t(p);
+            keys_out[at[k] + r] = 0xffffffffu;
+        }
     }
 }
 
@@ -225,24 +329,82 @@ __device__ void brick_segment_sort(uint32_t b, uint32_t beg, uint32_t n, uint32_
     }
 }
 
-// List of non-empty bricks: flags -> exclusive scan -> ids in increasing order.
-__global__ __launch_bounds__(256) void k_nonempty_flags(const uint32_t* count, int64_t nb,
-                                                        uint32_t* flags)
-{
-    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b <= nb) flags[b] = (b < nb && count[b] > 0u) ? 1u : 0u;
-}
+// brick_segment_sort for segments of at most SORT_REG * blockDim.x
+// indices: each index and its bin are gathered from memory once, into
+// registers; the bin count's LDS atomic returns the index's arrival rank,
+// which places it in its bin with no second pass over memory, and the bin
+// of every placed index is kept in LDS (ibin) for the rank step.
+constexpr int SORT_REG = 4;
+static_assert(SORT_REG * SORT_BT <= SORT_LDS_CAP, "register-path segments fit the LDS arrays");
 
-__global__ __launch_bounds__(256) void k_compact_active(const uint32_t* flags, const uint32_t* pos,
-                                                        int64_t nb, uint32_t* active)
+__device__ void brick_segment_sort_reg(uint32_t b, uint32_t beg, uint32_t n, uint32_t bvol,
+                                       const uint32_t* tmp, const uint32_t* keys_in,
+                                       uint32_t* items, uint32_t* ibin, uint32_t* cnt,
+                                       uint32_t* cur, uint32_t* perm, uint32_t* keys_out)
 {
-    const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (b < nb && flags[b]) active[1 + pos[b]] = uint32_t(b);
-    if (b == nb) active[0] = pos[nb];
+    const uint32_t kb = b * bvol;
+    uint32_t pv[SORT_REG], lb[SORT_REG], rk[SORT_REG];
+#pragma unroll
+    for (int r = 0; r < SORT_REG; ++r) {
+        const uint32_t i = threadIdx.x + uint32_t(r) * blockDim.x;
+        pv[r] = i < n ? tmp[beg + i] : 0u;
+    }
+    for (uint32_t l = threadIdx.x; l < bvol; l += blockDim.x) cnt[l] = 0;
+#pragma unroll
+    for (int r = 0; r < SORT_REG; ++r) {
+        const uint32_t i = threadIdx.x + uint32_t(r) * blockDim.x;
+        lb[r] = i < n ? keys_in[pv[r]] - kb : 0u;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_REG; ++r) {
+        const uint32_t i = threadIdx.x + uint32_t(r) * blockDim.x;
+        if (i < n) rk[r] = atomicAdd(&cnt[lb[r]], 1u);
+    }
+    __syncthreads();
+    // exclusive scan of cnt into cur (one wave, 64 bins at a time)
+    if (threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        uint32_t carry = 0;
+        for (uint32_t c0 = 0; c0 < bvol; c0 += 64) {
+            const uint32_t v = (c0 + lane < bvol) ? cnt[c0 + lane] : 0u;
+            uint32_t inc = v;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(inc, o);
+                if (lane >= o) inc += y;
+            }
+            if (c0 + lane < bvol) cur[c0 + lane] = carry + inc - v;
+            carry += __shfl(inc, 63);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < SORT_REG; ++r) {
+        const uint32_t i = threadIdx.x + uint32_t(r) * blockDim.x;
+        if (i < n) {
+            const uint32_t pos = cur[lb[r]] + rk[r];
+            items[pos] = pv[r];
+            ibin[pos] = lb[r];
+        }
+    }
+    __syncthreads();
+    // Indices are distinct, so the ranks inside a bin are a permutation of
+    // its slots.
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t v = items[i];
+        const uint32_t l = ibin[i];
+        const uint32_t s0 = cur[l], e = s0 + cnt[l];
+        uint32_t r = s0;
+        for (uint32_t j = s0; j < e; ++j) r += items[j] < v ? 1u : 0u;
+        perm[beg + r] = v;
+        keys_out[beg + r] = kb + l;
+    }
 }
 
 // Workgroups walk the non-empty bricks with a grid stride.  Dynamic LDS:
-// cnt[bvol] | cur[bvol] | items[SORT_LDS_CAP]; larger bricks use `big`.
+// cnt[bvol] | cur[bvol] | items[SORT_LDS_CAP] | ibin[SORT_LDS_CAP]; bricks
+// of more than SORT_LDS_CAP indices use `big`.
 __global__ __launch_bounds__(SORT_BT) void k_brick_sort(const uint32_t* start,
                                                         const uint32_t* active,
                                                         const uint32_t* tmp,
@@ -254,12 +416,18 @@ __global__ __launch_bounds__(SORT_BT) void k_brick_sort(const uint32_t* start,
     uint32_t* cnt = sort_lds;
     uint32_t* cur = sort_lds + bvol;
     uint32_t* lds_items = sort_lds + 2 * bvol;
+    uint32_t* ibin = lds_items + SORT_LDS_CAP;
     const uint32_t nact = active[0];
     for (uint32_t t = blockIdx.x; t < nact; t += gridDim.x) {
         const uint32_t b = active[1 + t];
         const uint32_t beg = start[b], n = start[b + 1] - beg;
-        uint32_t* items = n <= SORT_LDS_CAP ? lds_items : big + beg;
-        brick_segment_sort(b, beg, n, bvol, tmp, keys_in, items, cnt, cur, perm, keys_out);
+        if (n <= uint32_t(SORT_REG * SORT_BT)) {
+            brick_segment_sort_reg(b, beg, n, bvol, tmp, keys_in, lds_items, ibin, cnt, cur, perm,
+                                   keys_out);
+        } else {
+            uint32_t* items = n <= SORT_LDS_CAP ? lds_items : big + beg;
+            brick_segment_sort(b, beg, n, bvol, tmp, keys_in, items, cnt, cur, perm, keys_out);
+        }
         __syncthreads();
     }
 }
@@ -299,6 +467,15 @@ KeyGeom make_key_geom(const pic_mesh& mesh, const pic_binning& b)
     }
     g.rdx = 1.0 / mesh.h;
     g.dual = b.dual;
+    g.vshift = 0;
+    for (int d = 0; d < 3; ++d) {
+        const int e = b.brick[d];
+        int sh = 0;
+        while ((1 << sh) < e) ++sh;
+        g.bshift[d] = (1 << sh) == e ? sh : -1;
+        if (g.bshift[d] < 0) g.vshift = -1;
+        else if (g.vshift >= 0) g.vshift += sh;
+    }
     return g;
 }
 
@@ -318,54 +495,46 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
     DeviceGuard dg(ctx->device);
     if (!dg.ok) return PIC_ERR_HIP;
 
-    // scratch: count[nb+1] | cursor[nb+1] | scan tmp | act[nb+1] | keys_in[np] | tmp[w] | big[w]
-    // with w = max(np, nb+1) (tmp and big also hold the non-empty flags and
-    // their scan before the scatter).  act holds the active-brick list when
+    // scratch: count[nb+1] | cursor[nb+1] | tile sums (64-bit) | act[nb+1] |
+    // keys_in[np] | tmp[np] | big[np].  act holds the active-brick list when
     // the caller does not ask for it.
-    const int64_t scan_words = scan_tmp_words(nb + 1);
-    const int64_t head = (nb + 1) + (nb + 1) + scan_words + (active ? 0 : nb + 1);
-    const int64_t w = std::max<int64_t>(np, nb + 1);
-    const size_t bytes = size_t(head + np + 2 * w) * 4;
+    const int64_t ntile = (nb + 1 + SCAN_TILE - 1) / SCAN_TILE;
+    const int64_t head = (nb + 1) + (nb + 1) + 2 * ntile + (active ? 0 : nb + 1);
+    const size_t bytes = size_t(head + 3 * np) * 4;
     uint32_t* sc = static_cast<uint32_t*>(scratch(ctx, bytes));
     if (!sc) return PIC_ERR_ALLOC;
     uint32_t* count = sc;
     uint32_t* cursor = count + nb + 1;
-    uint32_t* scan_tmp = cursor + nb + 1;
-    uint32_t* act = active ? active : scan_tmp + scan_words;
+    uint64_t* tile_sums = reinterpret_cast<uint64_t*>(cursor + nb + 1);
+    uint32_t* act = active ? active : reinterpret_cast<uint32_t*>(tile_sums + ntile);
     uint32_t* keys_in = sc + head;
     uint32_t* tmp = keys_in + np;
-    uint32_t* big = tmp + w;
+    uint32_t* big = tmp + np;
     hipStream_t st = ctx->stream;
 
     TimedLaunch tl(ctx, "bin_sort");
     const unsigned fb = (unsigned)std::min<int64_t>((2 * nb + 2 + 255) / 256, 4096);
     k_fill_u32<<<fb, 256, 0, st>>>(count, 2 * nb + 2, 0u);
     if (np > 0) {
-        const unsigned b = blocks_for(np, 256);
+        const unsigned b = blocks_for(np, SORT_PT * SORT_ITEMS);
         if (dtype == PIC_F64)
-            k_cell_keys<double><<<b, 256, 0, st>>>(np, make_pfield<double>(*x), g, uint32_t(nb),
+            k_cell_keys<double><<<b, SORT_PT, 0, st>>>(np, make_pfield<double>(*x), g, uint32_t(nb),
                                                     keys_in, count,
                                                     ctx->d_err);
         else
-            k_cell_keys<float><<<b, 256, 0, st>>>(np, make_pfield<float>(*x), g, uint32_t(nb),
+            k_cell_keys<float><<<b, SORT_PT, 0, st>>>(np, make_pfield<float>(*x), g, uint32_t(nb),
                                                    keys_in, count,
                                                    ctx->d_err);
     }
-    exclusive_scan(st, count, bin_start, nb + 1, scan_tmp);
-    {
-        // non-empty brick list: flags and their scan live in tmp and big
-        // before the scatter uses them.
-        uint32_t* flags = tmp;
-        uint32_t* pos = big;
-        const unsigned fb2 = blocks_for(nb + 1, 256);
-        k_nonempty_flags<<<fb2, 256, 0, st>>>(count, nb, flags);
-        exclusive_scan(st, flags, pos, nb + 1, scan_tmp);
-        k_compact_active<<<fb2, 256, 0, st>>>(flags, pos, nb, act);
-    }
+    k_tile_sums<<<(unsigned)ntile, SCAN_THREADS, 0, st>>>(count, nb, tile_sums);
+    const bool prefixed = ntile > SCAN_TILE;
+    if (prefixed) k_scan_sums<<<1, SCAN_THREADS, 0, st>>>(tile_sums, ntile);
+    k_scan_compact<<<(unsigned)ntile, SCAN_THREADS, 0, st>>>(count, nb, tile_sums, prefixed,
+                                                             bin_start, act);
     if (np > 0) {
-        k_bin_scatter<<<blocks_for(np, 256), 256, 0, st>>>(np, keys_in, g.bvol, uint32_t(nb),
-                                                            bin_start, cursor, tmp, perm, keys);
-        const size_t lds = size_t(2 * g.bvol + SORT_LDS_CAP) * 4;
+        k_bin_scatter<<<blocks_for(np, SORT_PT * SORT_ITEMS), SORT_PT, 0, st>>>(
+            np, keys_in, g.vshift, g.bvol, uint32_t(nb), bin_start, cursor, tmp, perm, keys);
+        const size_t lds = size_t(2 * g.bvol + 2 * SORT_LDS_CAP) * 4;
         const unsigned nbk = persistent_blocks(ctx, k_brick_sort, SORT_BT, lds, nb);
         k_brick_sort<<<nbk, SORT_BT, lds, st>>>(bin_start, act, tmp, keys_in, g.bvol, big, perm,
                                                 keys);

@@ -14,11 +14,13 @@ struct KeyGeom {
     int32_t nb[3];
     uint32_t bvol;
     int32_t dual;
+    // log2 of the brick edges and volume when all are powers of two (the
+    // usual case), so the brick / bin split is shifts and masks; else -1.
+    int32_t bshift[3];
+    int32_t vshift;
 };
 
 KeyGeom make_key_geom(const pic_mesh& mesh, const pic_binning& b);
-int64_t scan_tmp_words(int64_t n);
-void exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, int64_t n, uint32_t* tmp);
 
 // Key of particle p (0xffffffff when outside the bins); *brick receives the
 // brick index.
@@ -40,8 +42,22 @@ __device__ __forceinline__ uint32_t cell_key(const KeyGeom& g, const PField<T>& 
         *brick = 0;
         return 0xffffffffu;
     }
-    const uint32_t bi = rel[0] / g.brick[0], bj = rel[1] / g.brick[1], bk = rel[2] / g.brick[2];
-    const uint32_t li = rel[0] % g.brick[0], lj = rel[1] % g.brick[1], lk = rel[2] % g.brick[2];
+    uint32_t bi, bj, bk, li, lj, lk;
+    if (g.vshift >= 0) {
+        bi = uint32_t(rel[0]) >> g.bshift[0];
+        bj = uint32_t(rel[1]) >> g.bshift[1];
+        bk = uint32_t(rel[2]) >> g.bshift[2];
+        li = uint32_t(rel[0]) & uint32_t(g.brick[0] - 1);
+        lj = uint32_t(rel[1]) & uint32_t(g.brick[1] - 1);
+        lk = uint32_t(rel[2]) & uint32_t(g.brick[2] - 1);
+    } else {
+        bi = rel[0] / g.brick[0];
+        bj = rel[1] / g.brick[1];
+        bk = rel[2] / g.brick[2];
+        li = rel[0] % g.brick[0];
+        lj = rel[1] % g.brick[1];
+        lk = rel[2] % g.brick[2];
+    }
     const uint32_t b = (bi * g.nb[1] + bj) * g.nb[2] + bk;
     *brick = b;
     return b * g.bvol + (li * g.brick[1] + lj) * g.brick[2] + lk;

@@ -173,6 +173,34 @@ def test_bin_sort_bit_exact(pa, wl0, brick, dt, dual):
     np.testing.assert_array_equal(gu.host(res.perm).view(np.uint32), perm)
     np.testing.assert_array_equal(gu.host(res.keys).view(np.uint32), keys[perm])
     np.testing.assert_array_equal(gu.host(res.bin_start).view(np.uint32), kstart[::bvol])
+    _check_active(gu, res, kstart[::bvol])
+
+
+def _check_active(gu, res, bin_start):
+    """The active list is [count, ids of the non-empty bricks ascending]."""
+    nonempty = np.flatnonzero(np.diff(bin_start.astype(np.int64)) > 0).astype(np.uint32)
+    act = gu.host(res.active).view(np.uint32)
+    assert act[0] == len(nonempty)
+    np.testing.assert_array_equal(act[1:1 + len(nonempty)], nonempty)
+
+
+def test_bin_sort_many_scan_tiles(pa, wl0):
+    """Over 2048 x 2048 bricks: the tile sums of the counts get a scan pass
+    of their own; perm, bin_start and the active list still match."""
+    import gpu_util as gu
+    x = gu.soa(wl0.x)
+    cell_lo, ncell = wl0.cell_region
+    big = (int(ncell[0]) + 300, int(ncell[1]) + 200, int(ncell[2]) + 60)
+    geom = pa.Binning(tuple(int(v) for v in cell_lo), big, (1, 1, 1), 0)
+    nb = pa.bin_count(geom)
+    assert nb > 2048 * 2048
+    res = pa.bin_sort(gu.ctx(), pa.Mesh(wl0.low, wl0.h), x, geom)
+    gu.ctx().synchronize()
+    keys, _ = orpic.cell_keys(wl0.x, wl0.low, wl0.h, geom.cell_lo, geom.ncell, (1, 1, 1), 0)
+    perm, kstart = orpic.bin_sort(keys, nb)
+    np.testing.assert_array_equal(gu.host(res.perm).view(np.uint32), perm)
+    np.testing.assert_array_equal(gu.host(res.bin_start).view(np.uint32), kstart)
+    _check_active(gu, res, kstart)
 
 
 def test_bin_sort_out_of_range_reported(pa, wl0):
