@@ -159,6 +159,8 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
     brick_origin<PB>(g, it.b, f0);
     const uint32_t kb = it.b * BINS;
 
+    // Work-item group l (four work-items, one per component) accumulates
+    // bin l of the brick.
     const int l = int(threadIdx.x) >> 2, comp = int(threadIdx.x) & 3;
     const int grp = comp > 0 ? comp : 1;
     const T hh = (T)a.lat.h;

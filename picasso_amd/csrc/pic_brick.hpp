@@ -68,13 +68,17 @@ __device__ __forceinline__ BrickItem load_item(const G& g, uint32_t t, uint32_t 
 
 // The items of the walk in one flat array, so that a workgroup resolves its
 // next brick with one load instead of the chain active -> bin_start.  Also
-// clears the stale count.
+// clears the walk counters and the stale count.
 template <int = 0>
 __global__ __launch_bounds__(256) void k_walk_items(const uint32_t* active,
                                                     const uint32_t* bin_start, uint32_t nbricks,
-                                                    BrickItem* items, uint32_t* stale)
+                                                    BrickItem* items, uint32_t* walk,
+                                                    uint32_t* stale)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) stale[0] = 0u;
+    if (blockIdx.x == 0) {
+        for (int i = threadIdx.x; i < WALK_LANES * WALK_PAD; i += blockDim.x) walk[i] = 0u;
+        if (threadIdx.x == 0) stale[0] = 0u;
+    }
     const uint32_t nw = active ? active[0] : nbricks;
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nw; t += gridDim.x * blockDim.x) {
         const uint32_t b = active ? active[1 + t] : t;
@@ -357,10 +361,9 @@ inline bool setup_walk(pic_context* ctx, int order, int entity, int64_t np, cons
     g.items = items;
     g.walk = w;
     g.stale = w + WALK_LANES * WALK_PAD;
-    if (fill_u32(ctx, w, WALK_LANES * WALK_PAD, 0u) != PIC_OK) return false;
     const unsigned nblk = unsigned(std::min<int64_t>((nb + 255) / 256, 2048));
     k_walk_items<<<nblk, 256, 0, ctx->stream>>>(bins.active, bins.bin_start, uint32_t(nb), items,
-                                                  g.stale);
+                                                  w, g.stale);
     return launch_status() == 0;
 }
 
