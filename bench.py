@@ -125,6 +125,31 @@ def time_steps(step, steps, warmup, ctx, families):
     return (t1 - t0) * 1e3 / steps, kt
 
 
+def time_graph_steps(step, steps, warmup, ctx):
+    """The same step captured once into a step graph (pic_graph_*) and
+    replayed K times: identical kernels and arguments, without the host's
+    per-launch gaps.  Runs on a stream of its own (capture needs one)."""
+    import torch
+    st = torch.cuda.Stream()
+    with torch.cuda.stream(st):
+        step.run()  # eager, on this stream: the context's scratch is sized
+        ctx.synchronize()
+        ctx.graph_begin()
+        step.run()
+        g = ctx.graph_end()
+        for _ in range(warmup):
+            g.launch()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.launch()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        ctx.synchronize()
+        g.close()
+    return (t1 - t0) * 1e3 / steps
+
+
 def cpu_baseline(wl, reps=3):
     """Oracle OpenMP FP64 step (atomic p2g) on the host cores."""
     from oracle import orpic
@@ -480,13 +505,17 @@ def main():
         log(f"fp32 first step from shuffled storage (sort, permute, step): "
             f"{shuffled_first_ms:.3f} ms")
     s32.reorder()
-    ms32, kt32 = time_steps(s32, args.steps, args.warmup, ctx, families)
-    log(f"fp32: {ms32:.3f} ms/step, kernels {json.dumps(kt32)}")
+    ms32_eager, kt32 = time_steps(s32, args.steps, args.warmup, ctx, families)
+    log(f"fp32: {ms32_eager:.3f} ms/step eager, kernels {json.dumps(kt32)}")
+    ms32 = time_graph_steps(s32, args.steps, args.warmup, ctx)
+    log(f"fp32: {ms32:.3f} ms/step as a step graph")
 
     s64 = Step(pa, ctx, wl, torch.float64, wl.x, wl.m, wl.c)
     s64.reorder()
-    ms64, kt64 = time_steps(s64, args.steps, args.warmup, ctx, families)
-    log(f"fp64: {ms64:.3f} ms/step, kernels {json.dumps(kt64)}")
+    ms64_eager, kt64 = time_steps(s64, args.steps, args.warmup, ctx, families)
+    log(f"fp64: {ms64_eager:.3f} ms/step eager, kernels {json.dumps(kt64)}")
+    ms64 = time_graph_steps(s64, args.steps, args.warmup, ctx)
+    log(f"fp64: {ms64:.3f} ms/step as a step graph")
     del s64
 
     # Error of the FP32 step against the FP64 step on the same rounded inputs.
@@ -579,13 +608,15 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded SplitMix64), inputs resident in HBM",
         "config": {"workload": f"APIC sphere r=0.3, {args.n}^3 cells x {args.ppc} ppc, "
-                               "quadratic node spline, cell-bin sort every step",
+                               "quadratic node spline, cell-bin sort every step, "
+                               "step replayed as a captured HIP graph",
                    "particles": P, "cells": args.n, "ppc": args.ppc,
                    "touched_nodes": n_touched, "brick": list(BRICK)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "ms_per_step_eager": ms32_eager,
         "fp64": {"particles_per_s": P / (ms64 * 1e-3), "ms_per_step": ms64,
-                 "kernels": kt64},
+                 "ms_per_step_eager": ms64_eager, "kernels": kt64},
         "fp32_over_fp64": ms64 / ms32,
         "fp32_step_ms_shuffled_storage": shuffled_ms,
         "fp32_first_step_ms_from_shuffled": shuffled_first_ms,
