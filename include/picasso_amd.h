@@ -233,6 +233,33 @@ int pic_g2p_gradu(pic_context* ctx, int order, int entity, int dtype,
                   const pic_grid* u_i, const pic_pfield* gradu,
                   const pic_bins* bins);
 
+/* Generic spline transfers: batch forms of Picasso::G2P::value / gradient /
+ * divergence (Picasso_ParticleInterpolation.hpp:110-189) and
+ * Picasso::P2G::value / gradient / divergence (:197-279), on any entity
+ * lattice and spline order.  ncomp counts the input's components; grid
+ * fields keep their components at stride cstride, particle fields at
+ * component stride.  With w_n the node weight and gw_n its gradient:
+ *   G2P VALUE       out(p, c)      = sum_n w_n f(n, c)            ncomp 1..3
+ *   G2P GRADIENT    out(p, 3c + d) = sum_n gw_n[d] f(n, c)        ncomp 1 or 3
+ *                   (the vector case is du_c/dx_d, as pic_g2p_gradu)
+ *   G2P DIVERGENCE  out(p)         = sum_n sum_d gw_n[d] f(n, d)  ncomp 3
+ *   P2G VALUE       f(n, c)       += w_n v(p, c)                  ncomp 1..3
+ *   P2G GRADIENT    f(n, d)       += gw_n[d] v(p)                 ncomp 1
+ *   P2G DIVERGENCE  f(n)          += sum_d gw_n[d] v(p, d)        ncomp 3
+ *                   f(n, i)       += sum_j gw_n[j] v(p, 3i + j)   ncomp 9
+ * P2G accumulates like a ScatterView (PIC_P2G_ZERO_FIRST zeroes the field
+ * first).  bins (optional, from pic_bin_sort) only sets the visiting
+ * order. */
+enum { PIC_INTERP_VALUE = 0, PIC_INTERP_GRADIENT = 1, PIC_INTERP_DIVERGENCE = 2 };
+int pic_g2p_interp(pic_context* ctx, int op, int order, int entity, int dtype,
+                   const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                   const pic_grid* field, int ncomp, const pic_pfield* out,
+                   const pic_bins* bins);
+int pic_p2g_interp(pic_context* ctx, int op, int order, int entity, int dtype,
+                   const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                   const pic_pfield* value, int ncomp, const pic_grid* field,
+                   uint32_t flags, const pic_bins* bins);
+
 /* F_p <- (I + dt*gradu) F_p, det(F_p) and F_p^{-1} in FP64
  * (Picasso_BatchedLinearAlgebra.hpp:3806-3834 product, :4335-4350 det,
  * :4624-4685 inverse).  gradu is of gradu_dtype; F, det, Finv are double. */

@@ -5,6 +5,8 @@
 //   Picasso::APIC::g2p(u_i, c_p, sd)                     (Picasso_APIC.hpp:407-413)
 //   Picasso::PolyPIC::p2g(m_p, u_p, c_p, mu_i, m_i, dt, sd) (Picasso_PolyPIC.hpp:39-50)
 //   Picasso::PolyPIC::g2p(u_i, c_p, sd)                  (Picasso_PolyPIC.hpp:235-243)
+//   Picasso::G2P::value / gradient / divergence, Picasso::P2G::value /
+//   gradient / divergence   (Picasso_ParticleInterpolation.hpp:110-279)
 //
 // The reference calls these once per particle inside a Kokkos kernel; here
 // each call covers every particle of the spline data at once (the batch form
@@ -138,6 +140,59 @@ inline void g2p_update_F(pic_context* ctx, const pic_grid* m_i, const pic_grid& 
 }
 
 }  // namespace APIC
+
+// Generic transfers, Picasso::G2P / P2G (Picasso_ParticleInterpolation.hpp:
+// 110-279): ncomp counts the input's components (see pic_g2p_interp).
+namespace G2P {
+
+inline void value(pic_context* ctx, const SplineData& sd, const pic_grid& field, int ncomp,
+                  const pic_pfield& result)
+{
+    check(pic_g2p_interp(ctx, PIC_INTERP_VALUE, sd.order, sd.entity, sd.dtype, &sd.mesh, sd.np,
+                         &sd.x, &field, ncomp, &result, sd.bins));
+}
+
+inline void gradient(pic_context* ctx, const SplineData& sd, const pic_grid& field, int ncomp,
+                     const pic_pfield& result)
+{
+    check(pic_g2p_interp(ctx, PIC_INTERP_GRADIENT, sd.order, sd.entity, sd.dtype, &sd.mesh,
+                         sd.np, &sd.x, &field, ncomp, &result, sd.bins));
+}
+
+inline void divergence(pic_context* ctx, const SplineData& sd, const pic_grid& field,
+                       const pic_pfield& result)
+{
+    check(pic_g2p_interp(ctx, PIC_INTERP_DIVERGENCE, sd.order, sd.entity, sd.dtype, &sd.mesh,
+                         sd.np, &sd.x, &field, 3, &result, sd.bins));
+}
+
+}  // namespace G2P
+
+namespace P2G {
+
+inline void value(pic_context* ctx, const SplineData& sd, const pic_pfield& value, int ncomp,
+                  const pic_grid& field, uint32_t flags = 0)
+{
+    check(pic_p2g_interp(ctx, PIC_INTERP_VALUE, sd.order, sd.entity, sd.dtype, &sd.mesh, sd.np,
+                         &sd.x, &value, ncomp, &field, flags, sd.bins));
+}
+
+inline void gradient(pic_context* ctx, const SplineData& sd, const pic_pfield& value,
+                     const pic_grid& field, uint32_t flags = 0)
+{
+    check(pic_p2g_interp(ctx, PIC_INTERP_GRADIENT, sd.order, sd.entity, sd.dtype, &sd.mesh,
+                         sd.np, &sd.x, &value, 1, &field, flags, sd.bins));
+}
+
+// Vector values (ncomp 3) or row-major tensor values (ncomp 9).
+inline void divergence(pic_context* ctx, const SplineData& sd, const pic_pfield& value,
+                       int ncomp, const pic_grid& field, uint32_t flags = 0)
+{
+    check(pic_p2g_interp(ctx, PIC_INTERP_DIVERGENCE, sd.order, sd.entity, sd.dtype, &sd.mesh,
+                         sd.np, &sd.x, &value, ncomp, &field, flags, sd.bins));
+}
+
+}  // namespace P2G
 
 namespace PolyPIC {
 

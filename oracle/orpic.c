@@ -556,6 +556,104 @@ int orc_g2p_gradu(int order, int entity, const double low[3], double h,
     return ORC_OK;
 }
 
+/* Generic spline transfers: Picasso::G2P::value / gradient / divergence
+ * and Picasso::P2G::value / gradient / divergence
+ * (Picasso_ParticleInterpolation.hpp:110-279), which forward to
+ * Cabana::Grid::G2P / P2G (Cabana 0.6.1, not in the reference tree).
+ * Restated from their definitions with w_n the node weight and gw_n its
+ * physical gradient (gw_n[d] = dw/dx_d, products of per-dimension factors
+ * as for orc_g2p_gradu):
+ *   G2P value       out(p, c)       = sum_n w_n f(n, c)
+ *   G2P gradient    out(p, 3c + d)  = sum_n gw_n[d] f(n, c)   (ncomp 1: out(p, d))
+ *   G2P divergence  out(p)          = sum_n sum_d gw_n[d] f(n, d)
+ *   P2G value       f(n, c)        += w_n v(p, c)
+ *   P2G gradient    f(n, d)        += gw_n[d] v(p)
+ *   P2G divergence  f(n)           += sum_d gw_n[d] v(p, d)          (vector)
+ *                   f(n, i)        += sum_j gw_n[j] v(p, 3i + j)     (tensor)
+ * The vector gradient's index order matches orc_g2p_gradu (du_i/dx_j at
+ * 3i + j).  Grid fields hold their components contiguously per node. */
+static void node_weights(const spl_t* sd, int i, int j, int k, double* w, double gw[3])
+{
+    *w = sd->w[0][i] * sd->w[1][j] * sd->w[2][k];
+    gw[0] = sd->g[0][i] * sd->w[1][j] * sd->w[2][k];
+    gw[1] = sd->w[0][i] * sd->g[1][j] * sd->w[2][k];
+    gw[2] = sd->w[0][i] * sd->w[1][j] * sd->g[2][k];
+}
+
+int orc_g2p_interp(int op, int order, int entity, const double low[3], double h,
+                   const int32_t lo[3], const int32_t n[3], int64_t np, const double* x,
+                   const double* f, int ncomp, double* out)
+{
+    int nout;
+    if (op == ORC_INTERP_VALUE && ncomp >= 1 && ncomp <= 3) nout = ncomp;
+    else if (op == ORC_INTERP_GRADIENT && (ncomp == 1 || ncomp == 3)) nout = 3 * ncomp;
+    else if (op == ORC_INTERP_DIVERGENCE && ncomp == 3) nout = 1;
+    else return ORC_ERR_ARG;
+    for (int64_t p = 0; p < np; ++p) {
+        spl_t sd;
+        int nk = spline(order, entity, low, h, x + 3 * p, &sd);
+        if (nk < 0) return nk;
+        if (!stencil_inside(&sd, lo, n)) return ORC_ERR_BOUNDS;
+        double r[9] = {0.0};
+        for (int i = 0; i < nk; ++i)
+            for (int j = 0; j < nk; ++j)
+                for (int k = 0; k < nk; ++k) {
+                    double w, gw[3];
+                    node_weights(&sd, i, j, k, &w, gw);
+                    const int64_t node =
+                        win_index(lo, n, sd.s[0][i], sd.s[1][j], sd.s[2][k]);
+                    const double* fn = f + node * ncomp;
+                    for (int c = 0; c < ncomp; ++c) {
+                        if (op == ORC_INTERP_VALUE) r[c] += w * fn[c];
+                        else if (op == ORC_INTERP_GRADIENT)
+                            for (int d = 0; d < 3; ++d) r[3 * c + d] += gw[d] * fn[c];
+                        else r[0] += gw[c] * fn[c];
+                    }
+                }
+        memcpy(out + nout * p, r, sizeof(double) * (size_t)nout);
+    }
+    return ORC_OK;
+}
+
+int orc_p2g_interp(int op, int order, int entity, const double low[3], double h,
+                   const int32_t lo[3], const int32_t n[3], int64_t np, const double* x,
+                   const double* v, int ncomp, double* f)
+{
+    int ngrid;
+    if (op == ORC_INTERP_VALUE && ncomp >= 1 && ncomp <= 3) ngrid = ncomp;
+    else if (op == ORC_INTERP_GRADIENT && ncomp == 1) ngrid = 3;
+    else if (op == ORC_INTERP_DIVERGENCE && (ncomp == 3 || ncomp == 9)) ngrid = ncomp / 3;
+    else return ORC_ERR_ARG;
+    for (int64_t p = 0; p < np; ++p) {
+        spl_t sd;
+        int nk = spline(order, entity, low, h, x + 3 * p, &sd);
+        if (nk < 0) return nk;
+        if (!stencil_inside(&sd, lo, n)) return ORC_ERR_BOUNDS;
+        const double* vp = v + ncomp * p;
+        for (int i = 0; i < nk; ++i)
+            for (int j = 0; j < nk; ++j)
+                for (int k = 0; k < nk; ++k) {
+                    double w, gw[3];
+                    node_weights(&sd, i, j, k, &w, gw);
+                    const int64_t node =
+                        win_index(lo, n, sd.s[0][i], sd.s[1][j], sd.s[2][k]);
+                    double* fn = f + node * ngrid;
+                    if (op == ORC_INTERP_VALUE) {
+                        for (int c = 0; c < ncomp; ++c) fn[c] += w * vp[c];
+                    } else if (op == ORC_INTERP_GRADIENT) {
+                        for (int d = 0; d < 3; ++d) fn[d] += gw[d] * vp[0];
+                    } else {
+                        for (int a = 0; a < ngrid; ++a) {
+                            double s = 0.0;
+                            for (int b = 0; b < 3; ++b) s += gw[b] * vp[3 * a + b];
+                            fn[a] += s;
+                        }
+                    }
+                }
+    }
+    return ORC_OK;
+}
+
 /* u_i = mu_i / m_i (doc/basic/explict_nearly_incompressible_flow.md:18-23
  * without external forces); nodes with no mass get u = 0. */
 int orc_grid_normalize(int64_t nn, int ncomp, const double* m,

@@ -98,6 +98,15 @@ int orc_polypic_g2p(int order, int entity, const double low[3], double h,
                     const int32_t lo[3], const int32_t n[3], int64_t np,
                     const double* x, const double* gu, int ncomp, double* c);
 
+/* Generic G2P / P2G transfers (value, gradient, divergence); see orpic.c. */
+enum { ORC_INTERP_VALUE = 0, ORC_INTERP_GRADIENT = 1, ORC_INTERP_DIVERGENCE = 2 };
+int orc_g2p_interp(int op, int order, int entity, const double low[3], double h,
+                   const int32_t lo[3], const int32_t n[3], int64_t np, const double* x,
+                   const double* f, int ncomp, double* out);
+int orc_p2g_interp(int op, int order, int entity, const double low[3], double h,
+                   const int32_t lo[3], const int32_t n[3], int64_t np, const double* x,
+                   const double* v, int ncomp, double* f);
+
 /* Velocity gradient at the particle, gradu[p][i][j] = du_i/dx_j, from a
  * collocated 3-component grid field (G2P::gradient,
  * Picasso_ParticleInterpolation.hpp:165-181). */

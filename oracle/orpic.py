@@ -52,6 +52,10 @@ def _declare(L):
                                   _D, _D, c_int, _D]
     L.orc_g2p_gradu.argtypes = [c_int, c_int, _D, c_double, _I32, _I32, c_i64,
                                 _D, _D, _D]
+    L.orc_g2p_interp.argtypes = [c_int, c_int, c_int, _D, c_double, _I32, _I32, c_i64, _D,
+                                 _D, c_int, _D]
+    L.orc_p2g_interp.argtypes = [c_int, c_int, c_int, _D, c_double, _I32, _I32, c_i64, _D,
+                                 _D, c_int, _D]
     L.orc_grid_normalize.argtypes = [c_i64, c_int, _D, _D, _D]
     L.orc_cell_keys.argtypes = [c_i64, _D, _D, c_double, _I32, _I32, _I32, c_int,
                                 _U32, _I32]
@@ -150,6 +154,36 @@ def g2p_gradu(order, entity, low, h, lo, n, x, gu):
                                _p(_i32(n), _I32), x.shape[0], _p(x), _p(gu),
                                _p(out)), "g2p_gradu")
     return out
+
+
+INTERP_VALUE, INTERP_GRADIENT, INTERP_DIVERGENCE = 0, 1, 2
+
+
+def g2p_interp(op, order, entity, low, h, lo, n, x, f, ncomp):
+    """G2P::value / gradient / divergence of a grid field f [nodes..., ncomp]
+    at the particles: out [P, nout] (nout = ncomp, 3*ncomp or 1)."""
+    x, f = _f64(x), _f64(f)
+    nout = {INTERP_VALUE: ncomp, INTERP_GRADIENT: 3 * ncomp, INTERP_DIVERGENCE: 1}[op]
+    out = np.zeros((x.shape[0], nout))
+    _check(lib().orc_g2p_interp(op, order, entity, _p(_f64(low)), h, _p(_i32(lo), _I32),
+                                _p(_i32(n), _I32), x.shape[0], _p(x), _p(f), ncomp,
+                                _p(out)), "g2p_interp")
+    return out
+
+
+def p2g_interp(op, order, entity, low, h, lo, n, x, v, ncomp, f=None):
+    """P2G::value / gradient / divergence of particle values v [P, ncomp],
+    accumulated into a grid field [nodes..., ngrid] (zeros when f is None)."""
+    x, v = _f64(x), _f64(v)
+    ngrid = {INTERP_VALUE: ncomp, INTERP_GRADIENT: 3, INTERP_DIVERGENCE: ncomp // 3}[op]
+    if f is None:
+        f = np.zeros(tuple(int(k) for k in n) + (ngrid,))
+    else:
+        f = _f64(f).copy()
+    _check(lib().orc_p2g_interp(op, order, entity, _p(_f64(low)), h, _p(_i32(lo), _I32),
+                                _p(_i32(n), _I32), x.shape[0], _p(x), _p(v), ncomp,
+                                _p(f)), "p2g_interp")
+    return f
 
 
 def grid_normalize(m, mu):

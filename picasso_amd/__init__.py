@@ -328,6 +328,79 @@ def g2p_gradu(ctx, u_i, gradu, sd):
     check(rc, "pic_g2p_gradu")
 
 
+INTERP_VALUE, INTERP_GRADIENT, INTERP_DIVERGENCE = 0, 1, 2
+
+
+def _g2p_interp(ctx, op, sd, field, ncomp, result, nout, what):
+    x = sd.x
+    mesh = sd.mesh.c()
+    rc = lib.pic_g2p_interp(ctx._bind_stream(), op, sd.order, sd.location, _dtype_code(x),
+                            ctypes.byref(mesh), x.shape[0], ctypes.byref(_pfield(x, 3, "x")),
+                            ctypes.byref(field.c()), ncomp,
+                            ctypes.byref(_pfield(result, nout, "result")), _bins_ptr(sd.bins))
+    check(rc, what)
+
+
+def _p2g_interp(ctx, op, sd, value, ncomp, field, zero_first, what):
+    x = sd.x
+    mesh = sd.mesh.c()
+    rc = lib.pic_p2g_interp(ctx._bind_stream(), op, sd.order, sd.location, _dtype_code(x),
+                            ctypes.byref(mesh), x.shape[0], ctypes.byref(_pfield(x, 3, "x")),
+                            ctypes.byref(_pfield(value, ncomp, "value")), ncomp,
+                            ctypes.byref(field.c()),
+                            _abi.P2G_ZERO_FIRST if zero_first else 0, _bins_ptr(sd.bins))
+    check(rc, what)
+
+
+class G2P:
+    """Picasso::G2P (Picasso_ParticleInterpolation.hpp:110-189) for every
+    particle of sd at once: result[p] from the grid field."""
+
+    @staticmethod
+    def value(ctx, sd, field, result, ncomp=None):
+        """result[p, c] = sum_n w_n field(n, c) (scalar: ncomp 1, vector: 3)."""
+        nc = ncomp if ncomp is not None else (1 if result.dim() == 1 else result.shape[1])
+        _g2p_interp(ctx, INTERP_VALUE, sd, field, nc, result, nc, "pic_g2p_interp(value)")
+
+    @staticmethod
+    def gradient(ctx, sd, field, result, ncomp=1):
+        """Scalar field (ncomp 1): result[p, d] = d field/dx_d; vector field
+        (ncomp 3): result[p, 3i + j] = d field_i/dx_j."""
+        _g2p_interp(ctx, INTERP_GRADIENT, sd, field, ncomp, result, 3 * ncomp,
+                    "pic_g2p_interp(gradient)")
+
+    @staticmethod
+    def divergence(ctx, sd, field, result):
+        """result[p] = div field (3-component field)."""
+        _g2p_interp(ctx, INTERP_DIVERGENCE, sd, field, 3, result, 1,
+                    "pic_g2p_interp(divergence)")
+
+
+class P2G:
+    """Picasso::P2G (Picasso_ParticleInterpolation.hpp:197-279) for every
+    particle of sd at once, accumulated into the grid field."""
+
+    @staticmethod
+    def value(ctx, sd, value, field, zero_first=False):
+        """field(n, c) += w_n value[p, c]."""
+        nc = 1 if value.dim() == 1 else value.shape[1]
+        _p2g_interp(ctx, INTERP_VALUE, sd, value, nc, field, zero_first, "pic_p2g_interp(value)")
+
+    @staticmethod
+    def gradient(ctx, sd, value, field, zero_first=False):
+        """field(n, d) += dw_n/dx_d value[p] (scalar values, 3-component field)."""
+        _p2g_interp(ctx, INTERP_GRADIENT, sd, value, 1, field, zero_first,
+                    "pic_p2g_interp(gradient)")
+
+    @staticmethod
+    def divergence(ctx, sd, value, field, zero_first=False):
+        """Vector values [np, 3]: field(n) += grad w_n . value[p]; tensor
+        values [np, 9] (row-major): field(n, i) += sum_j dw_n/dx_j value[p, 3i + j]."""
+        nc = value.shape[1]
+        _p2g_interp(ctx, INTERP_DIVERGENCE, sd, value, nc, field, zero_first,
+                    "pic_p2g_interp(divergence)")
+
+
 def update_F(ctx, gradu, dt, F, det, Finv):
     """F <- (I + dt gradu) F, det(F), F^{-1} in FP64."""
     rc = lib.pic_update_F(ctx._bind_stream(), F.shape[0], _dtype_code(gradu),

@@ -77,6 +77,10 @@ PROTOTYPES = {
                                     _c_double]),
     "pic_g2p_gradu": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(Mesh), _c_i64, _P(PField),
                                _P(Grid), _P(PField), _P(Bins)]),
+    "pic_g2p_interp": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _P(Mesh), _c_i64,
+                                _P(PField), _P(Grid), _c_int, _P(PField), _P(Bins)]),
+    "pic_p2g_interp": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _P(Mesh), _c_i64,
+                                _P(PField), _P(PField), _c_int, _P(Grid), _c_u32, _P(Bins)]),
     "pic_update_F": (_c_int, [_vp, _c_i64, _c_int, _P(PField), _c_double, _P(PField),
                               _P(PField), _P(PField)]),
     "pic_bin_count": (_c_i64, [_P(Binning)]),
