@@ -304,6 +304,47 @@ def config_polypic(pa, ctx, synth, steps=5):
             "kernels_ms": kt}
 
 
+def config_interp(pa, ctx, synth, steps=10):
+    """Generic transfers (P2G::value of a vector, G2P::gradient of a vector
+    field) on the 256^3 sphere, FP32, particles in bin order."""
+    import torch
+    wl = synth.apic_sphere(256, 8)
+    P = wl.np
+    dev = "cuda:0"
+    mesh = pa.Mesh(wl.low, wl.h)
+    x = torch.as_tensor(np.ascontiguousarray(wl.x.astype(np.float32).T), device=dev).t()
+    cell_lo, ncell = wl.cell_region
+    geom = pa.Binning(tuple(int(v) for v in cell_lo), tuple(int(v) + 1 for v in ncell), BRICK, 1)
+    bins = pa.bin_sort(ctx, mesh, x, geom)
+    ctx.synchronize()
+    xs = x.t()[:, bins.perm.long()].contiguous().t()  # particles stored in bin order
+    bins = pa.bin_sort(ctx, mesh, xs, geom)
+    v = torch.randn((3, P), device=dev).t()
+    grad = torch.empty((9, P), device=dev).t()
+    lo, n = wl.node_window
+    f = pa.GridField(torch.zeros(tuple(int(k) for k in n) + (3,), device=dev),
+                     tuple(int(k) for k in lo))
+    sd = pa.create_spline(pa.FieldLocation.Node, 2, mesh, xs, bins)
+
+    def step():
+        pa.P2G.value(ctx, sd, v, f, zero_first=True)
+        pa.G2P.gradient(ctx, sd, f, grad, ncomp=3)
+
+    step()
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    ms = _time(step, steps, 0)
+    ctx.set_timing(False)
+    ctx.synchronize()
+    kt = _kernel_avgs(ctx, ["zero_grid", "p2g_interp", "g2p_interp"])
+    ctx.reset_timing()
+    return {"workload": "P2G::value (vector) + G2P::gradient (vector) on the APIC sphere "
+                        "256^3 x 8 ppc, quadratic node spline, FP32, particles in bin order",
+            "particles": P, "ms_per_step": ms, "particles_per_s": P / (ms * 1e-3),
+            "kernels_ms": kt}
+
+
 def config_batched(pa, ctx, synth, n_inst=512, steps=3, lanes=4):
     """Independent instances: grid edge n_k = round(16 * 4^u_k), u_k ~ U(0, 1)
     (16^3 to 64^3 cells, log-uniform), ppc_k ~ U{2..16}, fields as the 32^3
@@ -581,6 +622,10 @@ def main():
         log("config: PolyPIC cubic 27 modes 128^3 FP32 ...")
         others["polypic_cubic27"] = config_polypic(pa, ctx, synth)
         log(f"  {others['polypic_cubic27']}")
+        torch.cuda.empty_cache()
+        log("config: generic transfers 256^3 FP32 ...")
+        others["generic_interp"] = config_interp(pa, ctx, synth)
+        log(f"  {others['generic_interp']}")
         torch.cuda.empty_cache()
         log("config: 512 batched instances ...")
         others["batched_instances"] = config_batched(pa, ctx, synth)

@@ -108,6 +108,10 @@ def _load():
         raise ImportError(
             f"{LIB_PATH} is missing: build the HIP library first "
             "(python -m picasso_amd.build or __graft_entry__.build())")
+    # torch first: its HIP runtime (same soname) then serves this library
+    # too.  Loaded the other way round, the library starts the system HIP
+    # runtime and torch's own copy finds no GPU.
+    import torch  # noqa: F401
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in PROTOTYPES.items():
         fn = getattr(lib, name)
