@@ -237,6 +237,9 @@ struct PolyBrickShape {
     static constexpr int STAGE = CAP * 16;
     static constexpr int SUMS = CPR * SLOTS;
     static constexpr int LDS = STAGE > SUMS ? STAGE : SUMS;
+    // Waves per SIMD the registers are held to: the cubic FP32 kernel fits
+    // two in 256 VGPRs without spilling; FP64 would spill, so it runs one.
+    static constexpr int WAVES = sizeof(T) == 4 ? 2 : 1;
 };
 
 // sum over exponents e of C[e0][e1][e2] y0^e0 y1^e1 y2^e2 (E exponents per
@@ -549,7 +552,7 @@ __global__ __launch_bounds__(256) void k_polypic_g2p_brick(PolyBrickArgs<T> g)
 }
 
 template <int ORDER, typename T>
-__global__ __launch_bounds__((PolyBrickShape<ORDER, T>::NT)) void k_polypic_p2g_brick(
+__global__ __launch_bounds__((PolyBrickShape<ORDER, T>::NT), (PolyBrickShape<ORDER, T>::WAVES)) void k_polypic_p2g_brick(
     PolyBrickArgs<T> g)
 {
     using S = PolyBrickShape<ORDER, T>;
