@@ -101,17 +101,6 @@ static bool dense_window(const pic_grid& g, int ncomp, int64_t* count)
     return true;
 }
 
-__global__ void k_fill_few_u32(uint32_t* p, int n, uint32_t v)
-{
-    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
-}
-
-int fill_u32(pic_context* ctx, uint32_t* p, int n, uint32_t v)
-{
-    k_fill_few_u32<<<1, 64, 0, ctx->stream>>>(p, n, v);
-    return launch_status();
-}
-
 int zero_grid(pic_context* ctx, int dtype, const pic_grid& g, int ncomp)
 {
     TimedLaunch tl(ctx, "zero_grid");

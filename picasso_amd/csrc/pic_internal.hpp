@@ -120,10 +120,6 @@ unsigned persistent_blocks(pic_context* ctx, K kernel, int threads, size_t dyn_l
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, items));
 }
 
-// Fill a few words of device memory on the stream (a kernel, so it can be
-// captured in graphs).
-int fill_u32(pic_context* ctx, uint32_t* p, int n, uint32_t v);
-
 // Zero a grid window on the stream (used by PIC_P2G_ZERO_FIRST).
 int zero_grid(pic_context* ctx, int dtype, const pic_grid& g, int ncomp);
 
