@@ -45,15 +45,20 @@ struct BrickArgs {
 //
 // LDS, two uses of one buffer:
 //  - particle staging: the brick's particles are copied in chunks of CAP
-//    with coalesced loads (consecutive work-items, consecutive particles),
-//    as four 4-scalar groups per particle: (f0, f1, f2, m) and, per
-//    momentum component d, (u_d, B_d0, B_d1, B_d2), where f is the
-//    fractional knot coordinate (knot base found once per particle here, in
-//    FP64, and checked against the bin; a mismatch is marked with f0 =
-//    STALE_F).  A bin group then reads two groups per particle with two
-//    wide LDS loads instead of issuing eight gathers to memory.  Groups are
-//    XOR-swizzled by (q >> 2) & 3 so 16 consecutive particles written at
-//    once hit distinct bank groups.
+//    with coalesced loads (consecutive work-items, consecutive particles).
+//    Each particle carries f (the fractional knot coordinate: knot base
+//    found once per particle here, in FP64, and checked against the bin; a
+//    mismatch is marked with f0 = STALE_F), m, and per momentum component d
+//    the row (u_d, B_d0, B_d1, B_d2).  A bin group reads (f, m) and its own
+//    component's row from LDS instead of issuing eight gathers to memory.
+//    FP32 keeps 16 rows of CAP scalars (f0..f2, m, then c_p in storage
+//    order) filled by direct global->LDS copies for m and c_p, so every
+//    load of a chunk is in flight at once without holding registers
+//    (measured 0.516 -> 0.502 ms on the 256^3 sphere).  FP64 stores four
+//    4-scalar groups per particle from registers, (f0, f1, f2, m) and the
+//    three momentum rows, read with one wide load each and XOR-swizzled by
+//    (q >> 2) & 3 so 16 consecutive particles written at once hit distinct
+//    bank groups.
 //  - bin sums, after the last chunk: [stencil offset][bin][component], the
 //    CPR components of a round contiguous, so a footprint node reads each
 //    (offset, bin) slot with one wide LDS load; staged in rounds of CPR
@@ -79,11 +84,20 @@ struct P2GShape {
     // two chunks for a typical brick (measured: 200 -> 1.46 ms, 256 -> 1.26
     // ms, 320 -> 1.42 ms on the 256^3 sphere).
     static constexpr int CAP = sizeof(T) == 4 ? 616 : 256;
+    // FP32 stages by direct global->LDS copies into rows of CAP scalars:
+    // f0..f2 (rows 0-2, written from registers), m (row ROW_M) and the 12
+    // scalars of c_p in storage order (rows ROW_C..ROW_C+11).
+    static constexpr bool DMA = sizeof(T) == 4;
+    static constexpr int ROW_M = 3, ROW_C = 4;
     static constexpr int STAGE = CAP * 16;                   // scalars
     static constexpr int SUMS = CPR * SLOTS;
     static constexpr int LDS = STAGE > SUMS ? STAGE : SUMS;
-    // Waves per SIMD the register allocation is held to.
-    static constexpr int WAVES = sizeof(T) == 8 && ORDER == 2 ? 3 : 1;
+    // Waves per SIMD the register allocation is held to: LDS admits four
+    // workgroups (four waves per SIMD) for FP32, and the bound keeps the
+    // quadratic kernel there (106 registers without spills; unbounded the
+    // compiler takes 144, which leaves room for three).
+    static constexpr int WAVES =
+        sizeof(T) == 8 ? (ORDER == 2 ? 3 : 1) : (ORDER == 2 ? 4 : ORDER == 3 ? 3 : 1);
 };
 
 // Accumulate one particle's component into the bin's stencil sums.  v and
@@ -172,6 +186,60 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
     for (uint32_t c0 = beg; c0 < end; c0 += CAP) {
         const uint32_t c1 = min(end, c0 + uint32_t(CAP));
         uint32_t* rng = st.begin_chunk(g, a.perm, beg, c0, c1);
+        if constexpr (S::DMA) {
+            // Positions into registers, m and B_p straight into their LDS
+            // rows: every load of the chunk is in flight before any result
+            // is used, with no registers held for the 13 copied scalars.
+            const int wq = int(threadIdx.x) & ~63;  // the wave's first position
+            // The slot indices come either from the prefetch or from loads
+            // just issued; one wait here lets the compiler count the loads
+            // below exactly instead of waiting before each slot's batch.
+            __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+            T xs[K][3];
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (c0 + threadIdx.x + uint32_t(k * NT) >= c1) continue;
+                const int64_t xb = a.x.base(int64_t(st.cs.perm[k]));
+#pragma unroll
+                for (int dim = 0; dim < 3; ++dim) xs[k][dim] = a.x.at(xb, dim);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                if (c0 + threadIdx.x + uint32_t(k * NT) >= c1) continue;
+                const int64_t p = int64_t(st.cs.perm[k]);
+                const int64_t cb = a.c.base(p);
+                T* row = lds + (wq + k * NT);
+                load_to_lds(&a.m.at(a.m.base(p), 0), row + S::ROW_M * CAP);
+#pragma unroll
+                for (int e = 0; e < 12; ++e) load_to_lds(&a.c.at(cb, e), row + (S::ROW_C + e) * CAP);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                const uint32_t s = c0 + threadIdx.x + uint32_t(k * NT);
+                if (s >= c1) continue;
+                const int q = int(s - c0);
+                const int lb = int(st.cs.key[k] - kb);
+                const int eb[3] = {f0[0] + lb / (PB * PB), f0[1] + (lb / PB) % PB, f0[2] + lb % PB};
+                T fr[3];
+                bool ok = true;
+#pragma unroll
+                for (int dim = 0; dim < 3; ++dim) {
+                    double fd;
+                    const double kb0 = knot_base<ORDER>(double(xs[k][dim]), a.lat.o[dim],
+                                                         a.lat.rdx, fd);
+                    ok = ok && int(kb0) + Knots<ORDER>::FIRST == eb[dim];
+                    fr[dim] = T(fd);
+                }
+                if (!ok) {
+                    push_stale(g, s);
+                    fr[0] = T(STALE_F);
+                }
+#pragma unroll
+                for (int dim = 0; dim < 3; ++dim) lds[dim * CAP + q] = fr[dim];
+                st.cs.template note<BINS>(k, q, kb, rng);
+            }
+            __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+        } else {
         // Stage the chunk: consecutive work-items, consecutive particles.
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -210,14 +278,31 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
             }
             st.cs.template note<BINS>(k, q, kb, rng);
         }
+        }
         st.end_chunk();
         __syncthreads();
         if (c0 == beg) w.fetch();
         const uint32_t q0 = rng[l], q1 = rng[BINS + l];
         for (uint32_t q = q0; q < q1; ++q) {
-            const T4 fm = *reinterpret_cast<const T4*>(lds + stage_at(int(q), 0));
-            if (fm.x == T(STALE_F)) continue;
-            T4 ub = *reinterpret_cast<const T4*>(lds + stage_at(int(q), grp));
+            T4 fm, ub;
+            if constexpr (S::DMA) {
+                fm.x = lds[q];
+                if (fm.x == T(STALE_F)) continue;
+                fm.y = lds[CAP + q];
+                fm.z = lds[2 * CAP + q];
+                fm.w = lds[S::ROW_M * CAP + q];
+                // u_d = c(0, d), B(d, a) = c(1 + a, d): rows d, 3 + d, 6 + d,
+                // 9 + d of the c block.
+                const T* cr = lds + (S::ROW_C + grp - 1) * CAP + q;
+                ub.x = cr[0];
+                ub.y = cr[3 * CAP];
+                ub.z = cr[6 * CAP];
+                ub.w = cr[9 * CAP];
+            } else {
+                fm = *reinterpret_cast<const T4*>(lds + stage_at(int(q), 0));
+                if (fm.x == T(STALE_F)) continue;
+                ub = *reinterpret_cast<const T4*>(lds + stage_at(int(q), grp));
+            }
             if (comp == 0) {
                 ub.x = T(1);
                 ub.y = ub.z = ub.w = T(0);

@@ -238,6 +238,22 @@ struct BrickStage {
     }
 };
 
+// Direct copy of one 4-byte scalar per active work-item from global memory
+// into LDS, without a register: lane i of the wave writes wave_dst + 4 i
+// (wave_dst must be the same for the whole wave; lanes masked off write
+// nothing).  Completion is counted by the vector-memory counter, so readers
+// in other waves need a vmcnt(0) wait before the barrier they sync on.
+__device__ __forceinline__ void load_to_lds(const void* src, void* wave_dst)
+{
+    __builtin_amdgcn_global_load_lds(
+        (const void __attribute__((address_space(1)))*)src,
+        (void __attribute__((address_space(3)))*)wave_dst, 4, 0, 0);
+}
+
+// s_waitcnt operand for vmcnt(0), other counters left free (gfx9 layout:
+// vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
+constexpr int WAIT_VMCNT0 = 0x0F70;
+
 template <typename G>
 __device__ __forceinline__ void push_stale(const G& g, uint32_t s)
 {
