@@ -113,11 +113,16 @@ typedef struct pic_pfield {
 } pic_pfield;
 
 /* Binning geometry.  Bins are the cells of a lattice aligned with the mesh:
- * bin index c = floor((x - low)/h + dual/2) per dimension, so dual = 0 bins
- * by mesh cell and dual = 1 by node-centred (dual) cell -- the stencil base
- * of quadratic splines on nodes.  Bins [cell_lo, cell_lo+ncell) are grouped
- * in bricks of brick[3] bins; key = brick * brick_volume + bin_in_brick,
- * both linearised i-major (k fastest). */
+ * bin index c = floor((x - low)/h + dual_d/2) in dimension d, so dual_d = 0
+ * bins by mesh cell and dual_d = 1 by node-centred (dual) cell -- the
+ * stencil base of quadratic splines on nodes.  dual = 0: no dimension is
+ * dual; dual = 1: all three are; dual = PIC_DUAL_DIMS | mask: dimension d
+ * is dual when bit d of mask is set (staggered Face/Edge lattices, whose
+ * knot bases are node-like in some dimensions and cell-like in others).
+ * Bins [cell_lo, cell_lo+ncell) are grouped in bricks of brick[3] bins;
+ * key = brick * brick_volume + bin_in_brick, both linearised i-major (k
+ * fastest). */
+#define PIC_DUAL_DIMS 0x10
 typedef struct pic_binning {
     int32_t cell_lo[3];
     int32_t ncell[3];

@@ -701,6 +701,12 @@ int orc_cell_keys(int64_t np, const double* x, const double low[3], double h,
                   int32_t* cells)
 {
     if (!(h > 0.0)) return ORC_ERR_ARG;
+    /* dual: 0 none, 1 every dimension, 0x10 | mask per dimension. */
+    int dmask;
+    if (dual == 0) dmask = 0;
+    else if (dual == 1) dmask = 7;
+    else if ((dual & ~7) == 0x10) dmask = dual & 7;
+    else return ORC_ERR_ARG;
     int64_t nb[3], bvol = 1, total = 1;
     for (int d = 0; d < 3; ++d) {
         if (ncell[d] <= 0 || brick[d] <= 0) return ORC_ERR_ARG;
@@ -713,7 +719,7 @@ int orc_cell_keys(int64_t np, const double* x, const double low[3], double h,
     for (int64_t p = 0; p < np; ++p) {
         int64_t rel[3];
         for (int d = 0; d < 3; ++d) {
-            const double t = (x[3 * p + d] - low[d]) * rdx + (dual ? 0.5 : 0.0);
+            const double t = (x[3 * p + d] - low[d]) * rdx + (((dmask >> d) & 1) ? 0.5 : 0.0);
             const int64_t c = (int64_t)floor(t);
             if (cells) cells[3 * p + d] = (int32_t)c;
             rel[d] = c - cell_lo[d];

@@ -118,8 +118,10 @@ int orc_g2p_gradu(int order, int entity, const double low[3], double h,
 int orc_grid_normalize(int64_t nn, int ncomp, const double* m,
                        const double* mu, double* u);
 
-/* Cell keys.  cell = floor((x - low) * (1/h) + dual/2) (dual = 0: mesh
- * cells, dual = 1: node-centred dual cells), relative to cell_lo, must lie
+/* Cell keys.  cell = floor((x - low) * (1/h) + dual_d/2) per dimension d
+ * (dual_d = 0: mesh cells, 1: node-centred dual cells; dual = 0: none,
+ * 1: every dimension, 0x10 | mask: dimension d when bit d is set, as
+ * pic_binning), relative to cell_lo, must lie
  * in [0, ncell).  Cells are grouped in bricks of brick[3] cells;
  * key = brick_linear * brick_volume + cell_in_brick_linear, both
  * linearisations i-major (k fastest).  brick = {1,1,1} gives the plain

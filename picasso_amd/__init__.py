@@ -28,7 +28,7 @@ from ._abi import (CELL, EDGE_I, EDGE_J, EDGE_K, F32, F64, FACE_I, FACE_J, FACE_
 
 __all__ = ["Context", "Mesh", "GridField", "create_spline", "APIC", "PolyPIC", "FieldLocation",
            "grid_normalize", "g2p_gradu", "update_F", "bin_sort", "bin_count", "Binning",
-           "BinSort", "PicError",
+           "BinSort", "PicError", "DUAL_DIMS", "aligned_dual",
            "lib", "LIB_PATH"]
 
 LIB_PATH = _abi.LIB_PATH
@@ -46,6 +46,42 @@ class FieldLocation:
     @staticmethod
     def Edge(dim):
         return EDGE_I + dim
+
+
+def _cell_centred(entity, d):
+    """Whether the entity's lattice is cell-centred in dimension d (nodes:
+    never; cells: always; Face<D>: all but D; Edge<D>: only D)."""
+    if entity == NODE:
+        return False
+    if entity == CELL:
+        return True
+    if FACE_I <= entity <= FACE_K:
+        return d != entity - FACE_I
+    return d == entity - EDGE_I
+
+
+DUAL_DIMS = 0x10  # pic_binning dual = DUAL_DIMS | mask: dimension d dual when bit d is set
+
+
+def aligned_dual(order, entity):
+    """The pic_binning dual value whose bins are the knot-base lattice of
+    splines of `order` on `entity`, so that the brick kernels apply: per
+    dimension, dual (node-centred bins) exactly when the spline is quadratic
+    on a node-aligned dimension or linear/cubic on a cell-centred one."""
+    mask = 0
+    for d in range(3):
+        if (order == 2) != _cell_centred(entity, d):
+            mask |= 1 << d
+    return 0 if mask == 0 else 1 if mask == 7 else DUAL_DIMS | mask
+
+
+def dual_dims(dual):
+    """The per-dimension flags (0/1, three of them) of a pic_binning dual."""
+    if dual in (0, 1):
+        return (dual,) * 3
+    if dual & ~7 != DUAL_DIMS:
+        raise ValueError(f"dual {dual:#x} is not 0, 1 or DUAL_DIMS | mask")
+    return tuple((dual >> d) & 1 for d in range(3))
 
 
 def _dtype_code(t):
@@ -412,8 +448,11 @@ def update_F(ctx, gradu, dt, F, det, Finv):
 
 @dataclass
 class Binning:
-    """Bin lattice (pic_binning): bins floor((x-low)/h + dual/2) in
-    [cell_lo, cell_lo+ncell), grouped in bricks of `brick` bins."""
+    """Bin lattice (pic_binning): bins floor((x-low)/h + dual_d/2) in
+    [cell_lo, cell_lo+ncell), grouped in bricks of `brick` bins.  dual is 0
+    (mesh cells), 1 (node-centred cells in every dimension) or
+    DUAL_DIMS | mask (node-centred in the dimensions whose bit is set);
+    aligned_dual(order, entity) gives the one the brick kernels need."""
     cell_lo: Sequence[int]
     ncell: Sequence[int]
     brick: Sequence[int] = (8, 8, 8)

@@ -158,7 +158,7 @@ using P2GState = BrickStage<(P2GShape<ORDER, T>::CAP + P2GShape<ORDER, T>::NT - 
                                 P2GShape<ORDER, T>::NT,
                             P2GShape<ORDER, T>::NT, P2GShape<ORDER, T>::BINS>;
 
-template <int ORDER, typename T>
+template <int ORDER, typename T, bool STAG>
 __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const BrickItem& it,
                                               BrickWalk<true, BrickArgs<T>>& w, T* lds,
                                               P2GState<ORDER, T>& st)
@@ -177,7 +177,9 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
     // Work-item group l (four work-items, one per component) accumulates
     // bin l of the brick.
     const int l = int(threadIdx.x) >> 2, comp = int(threadIdx.x) & 3;
-    const int grp = comp > 0 ? comp : 1;
+    // Staggered: every momentum work-item works on component sdim (the
+    // one grid component), and only mass and that sum are written.
+    const int grp = STAG ? 1 + a.sdim : (comp > 0 ? comp : 1);
     const T hh = (T)a.lat.h;
     const T Dinv = T(ORDER == 3 ? 3.0 : 4.0) / (hh * hh);  // APIC::inertialScaling (:33-47)
     T acc[NN];
@@ -325,12 +327,12 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
         if (r == 0) w.publish();
         __syncthreads();
         if (r == 0) st.prefetch_next(g, a.perm, w, CAP);
-        gather_flush<NK, PB, NT, S::CPR>(lds, f0, r_lo, 1, a.gm, a.gmu, a.err);
+        gather_flush<NK, PB, NT, S::CPR>(lds, f0, r_lo, 1, a.gm, a.gmu, a.err, STAG ? 2 : 4);
         if (r + 1 < S::NROUND) __syncthreads();
     }
 }
 
-template <int ORDER, typename T>
+template <int ORDER, typename T, bool STAG>
 __global__ __launch_bounds__((P2GShape<ORDER, T>::NT), (P2GShape<ORDER, T>::WAVES)) void k_apic_p2g_brick(
     BrickArgs<T> g)
 {
@@ -341,11 +343,11 @@ __global__ __launch_bounds__((P2GShape<ORDER, T>::NT), (P2GShape<ORDER, T>::WAVE
     for (int i = threadIdx.x; i < 4 * S::BINS; i += S::NT) rng[i] = 0u;
     P2GState<ORDER, T> st{rng, 0, false, {}};
     walk_bricks<true>(g, q, [&](const BrickItem& it, BrickWalk<true, BrickArgs<T>>& w) {
-        p2g_one_brick<ORDER, T>(g, it, w, lds, st);
+        p2g_one_brick<ORDER, T, STAG>(g, it, w, lds, st);
     });
 }
 
-template <int BB, int ORDER, typename T, bool GF>
+template <int BB, int ORDER, typename T, bool GF, bool STAG>
 __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const BrickItem& it,
                                               BrickWalk<false, BrickArgs<T>>& w, T* us)
 {
@@ -363,9 +365,21 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const Brick
         const int64_t off = in ? gu.off(ni, nj, nk) : 0;
         const int64_t moff = in && a.gnm.data ? a.gnm.off(ni, nj, nk) : 0;
         T u[3] = {T(0), T(0), T(0)};
-        if (in) node_velocity3(gu, a.gnm, off, moff, u);
+        if constexpr (STAG) {
+            // One scalar grid (the staggered component).
+            if (in) {
+                u[0] = gu.data[off];
+                if (a.gnm.data) {
+                    const T mm = a.gnm.data[moff];
+                    u[0] = T(double(u[0]) * (mm > T(0) ? 1.0 / double(mm) : 0.0));
+                }
+            }
+            us[q] = u[0];
+        } else {
+            if (in) node_velocity3(gu, a.gnm, off, moff, u);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) us[d * FV + q] = u[d];
+            for (int d = 0; d < 3; ++d) us[d * FV + q] = u[d];
+        }
     }
     w.publish();
     __syncthreads();
@@ -401,11 +415,13 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const Brick
             }
             const int lbase = (li * FN + lj) * FN + lk;
             T cp[12], L[9];
-            apic_g2p_particle<ORDER, false, GF>(a, sd, [&](int i, int j, int k, T* ue) {
+            apic_g2p_particle<ORDER, STAG, GF>(a, sd, [&](int i, int j, int k, T* ue) {
                 const int q = lbase + (i * FN + j) * FN + k;
                 ue[0] = us[q];
-                ue[1] = us[FV + q];
-                ue[2] = us[2 * FV + q];
+                if constexpr (!STAG) {
+                    ue[1] = us[FV + q];
+                    ue[2] = us[2 * FV + q];
+                }
             }, cp, L);
             const int64_t cb = a.c.base(pp[r]);
 #pragma unroll
@@ -415,14 +431,14 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const Brick
     }
 }
 
-template <int BB, int ORDER, typename T, bool GF>
+template <int BB, int ORDER, typename T, bool GF, bool STAG>
 __global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
 {
     constexpr int FN = BB + ORDER;
-    __shared__ T us[3 * FN * FN * FN];
+    __shared__ T us[(STAG ? 1 : 3) * FN * FN * FN];
     __shared__ BrickItem q[2];
     walk_bricks<false>(g, q, [&](const BrickItem& it, BrickWalk<false, BrickArgs<T>>& w) {
-        g2p_one_brick<BB, ORDER, T, GF>(g, it, w, us);
+        g2p_one_brick<BB, ORDER, T, GF, STAG>(g, it, w, us);
     });
 }
 
@@ -442,12 +458,20 @@ template <typename T>
 struct P2GBrickLaunch {
     pic_context* ctx;
     const BrickArgs<T>& g;
+    bool stag;
+    template <int O, bool STAG>
+    void launch()
+    {
+        constexpr int NT = P2GShape<O, T>::NT;
+        const unsigned nbk =
+            persistent_blocks(ctx, k_apic_p2g_brick<O, T, STAG>, NT, 0, g.nbricks);
+        k_apic_p2g_brick<O, T, STAG><<<nbk, NT, 0, ctx->stream>>>(g);
+    }
     template <int O>
     void go()
     {
-        constexpr int NT = P2GShape<O, T>::NT;
-        const unsigned nbk = persistent_blocks(ctx, k_apic_p2g_brick<O, T>, NT, 0, g.nbricks);
-        k_apic_p2g_brick<O, T><<<nbk, NT, 0, ctx->stream>>>(g);
+        if (stag) launch<O, true>();
+        else launch<O, false>();
     }
     void by_order(int order)
     {
@@ -461,18 +485,21 @@ template <typename T>
 struct G2PBrickLaunch {
     pic_context* ctx;
     const BrickArgs<T>& g;
-    template <int BB, int O, bool GF>
+    bool stag;
+    template <int BB, int O, bool GF, bool STAG>
     void launch()
     {
         const unsigned nbk =
-            persistent_blocks(ctx, k_apic_g2p_brick<BB, O, T, GF>, BT, 0, g.nbricks);
-        k_apic_g2p_brick<BB, O, T, GF><<<nbk, BT, 0, ctx->stream>>>(g);
+            persistent_blocks(ctx, k_apic_g2p_brick<BB, O, T, GF, STAG>, BT, 0, g.nbricks);
+        k_apic_g2p_brick<BB, O, T, GF, STAG><<<nbk, BT, 0, ctx->stream>>>(g);
     }
+    // The F update is collocated only (the caller rejects it otherwise).
     template <int BB, int O>
     void go()
     {
-        if (g.a.F.data) launch<BB, O, true>();
-        else launch<BB, O, false>();
+        if (stag) launch<BB, O, false, true>();
+        else if (g.a.F.data) launch<BB, O, true, false>();
+        else launch<BB, O, false, false>();
     }
     template <int BB>
     void by_order(int order)
@@ -495,7 +522,7 @@ static int p2g_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
         return PIC_ERR_UNSUPPORTED;
     {
         TimedLaunch tl(ctx, "apic_p2g");
-        P2GBrickLaunch<T>{ctx, g}.by_order(order);
+        P2GBrickLaunch<T>{ctx, g, stag_dim(entity) >= 0}.by_order(order);
         if (launch_status()) return PIC_ERR_HIP;
     }
     return apic_p2g_list(ctx, order, entity, dtype, mesh, np, x, m, c, gm, gmu, bins.perm, g.stale);
@@ -506,6 +533,7 @@ static int g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
                          const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
                          const pic_pfield& c, const pic_bins& bins, int dtype, const FUpdate* fu)
 {
+    if (fu && stag_dim(entity) >= 0) return PIC_ERR_UNSUPPORTED;
     BrickArgs<T> g;
     if (!make_brick_args<T>(ctx, order, entity, mesh, np, x, nullptr, c, nullptr, gu, gmass, bins,
                             g))
@@ -513,7 +541,7 @@ static int g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
     set_F_update(g.a, fu);
     {
         TimedLaunch tl(ctx, "apic_g2p");
-        G2PBrickLaunch<T> L{ctx, g};
+        G2PBrickLaunch<T> L{ctx, g, stag_dim(entity) >= 0};
         if (brick_edge(bins.geom) == 4) L.template by_order<4>(order);
         else L.template by_order<8>(order);
         if (launch_status()) return PIC_ERR_HIP;

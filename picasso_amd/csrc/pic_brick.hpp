@@ -28,7 +28,7 @@ struct alignas(16) BrickItem {
     uint32_t nbricks;                                                                \
     int32_t cell_lo[3];                                                              \
     int32_t nb[3];                                                                   \
-    int32_t knot_off; /* first knot index = bin index + knot_off */
+    int32_t knot_off[3]; /* first knot index = bin index + knot_off, per dimension */
 
 // The dynamic walk draws walk indices from WALK_LANES counters: lane x hands
 // out the indices congruent to x modulo WALK_LANES, in order.  Every draw on
@@ -45,9 +45,9 @@ __device__ __forceinline__ void brick_origin(const G& g, uint32_t b, int f0[3])
     const int bk = int(b % uint32_t(g.nb[2]));
     const int bj = int((b / uint32_t(g.nb[2])) % uint32_t(g.nb[1]));
     const int bi = int(b / (uint32_t(g.nb[2]) * uint32_t(g.nb[1])));
-    f0[0] = g.cell_lo[0] + bi * BB + g.knot_off;
-    f0[1] = g.cell_lo[1] + bj * BB + g.knot_off;
-    f0[2] = g.cell_lo[2] + bk * BB + g.knot_off;
+    f0[0] = g.cell_lo[0] + bi * BB + g.knot_off[0];
+    f0[1] = g.cell_lo[1] + bj * BB + g.knot_off[1];
+    f0[2] = g.cell_lo[2] + bk * BB + g.knot_off[2];
 }
 
 // Walk indices run over the active list when there is one, else over every
@@ -280,10 +280,11 @@ __device__ __forceinline__ int stage_at(int q, int g)
 // brick whose first knot is f0.  Each footprint node adds the sums landing
 // on it in FP64 and applies one atomic per component: component r_lo + cc
 // goes to gm (cc index 0 of round 0, when gm is not null) or to component
-// (r_lo + cc - c_mom) of gmu.
+// (r_lo + cc - c_mom) of gmu; components from nwrite on are not written.
 template <int NK, int PB, int NT, int CPR, typename T>
 __device__ __forceinline__ void gather_flush(const T* sums, const int f0[3], int r_lo, int c_mom,
-                                             const GView<T>& gm, const GView<T>& gmu, int* err)
+                                             const GView<T>& gm, const GView<T>& gmu, int* err,
+                                             int nwrite = 1 << 30)
 {
     constexpr int FN = PB + NK - 1;
     constexpr int FV = FN * FN * FN;
@@ -320,22 +321,29 @@ __device__ __forceinline__ void gather_flush(const T* sums, const int f0[3], int
         for (int cc = 0; cc < CPR; ++cc) {
             const int c = r_lo + cc;
             const T v = T(tot[cc]);
-            if (v == T(0)) continue;
+            if (v == T(0) || c >= nwrite) continue;
             if (c < c_mom) atomic_add(gm.data + gm.off(ni, nj, nk), v);
             else atomic_add(gmu.data + off + (c - c_mom) * gmu.cs, v);
         }
     }
 }
 
-// Knot offset of the bins for (order, entity), or false when the bin
-// lattice is not the spline's knot-base lattice.  In half cells: the base is
-// floor(t_node - centred + order2) and the bin floor(t_node + dual).
-inline bool aligned_knot_offset(int order, int entity, int dual, int* knot_off)
+// Knot offset of the bins for (order, entity) per dimension, or false when
+// the bin lattice is not the spline's knot-base lattice in every dimension.
+// In half cells: the base is floor(t_node - centred + order2) and the bin
+// floor(t_node + dual), with centred and dual taken per dimension (staggered
+// Face/Edge lattices are node-like in some dimensions, cell-like in the
+// others).
+inline bool aligned_knot_offset(int order, int entity, int dual, int32_t knot_off[3])
 {
-    if (entity != PIC_NODE && entity != PIC_CELL) return false;
-    const int halves = (order == 2 ? 1 : 0) - (entity == PIC_CELL ? 1 : 0) - (dual ? 1 : 0);
-    if (halves != 0 && halves != -2) return false;
-    *knot_off = halves / 2 + (order == 1 ? 0 : -1);
+    const int mask = dual_mask(dual);
+    if (mask < 0) return false;
+    for (int d = 0; d < 3; ++d) {
+        const int halves =
+            (order == 2 ? 1 : 0) - (cell_centred(entity, d) ? 1 : 0) - ((mask >> d) & 1);
+        if (halves != 0 && halves != -2) return false;
+        knot_off[d] = halves / 2 + (order == 1 ? 0 : -1);
+    }
     return true;
 }
 
@@ -356,7 +364,7 @@ inline bool setup_walk(pic_context* ctx, int order, int entity, int64_t np, cons
 {
     const int bb = brick_edge(bins.geom);
     if (!bb || !bins.keys || !bins.bin_start || !bins.perm) return false;
-    if (!aligned_knot_offset(order, entity, bins.geom.dual, &g.knot_off)) return false;
+    if (!aligned_knot_offset(order, entity, bins.geom.dual, g.knot_off)) return false;
     const int64_t nb = pic_bin_count(&bins.geom);
     if (nb <= 0) return false;
     g.keys = bins.keys;

@@ -209,6 +209,17 @@ __device__ __forceinline__ double knot_base(double x, double o, double rdx,
     return b;
 }
 
+// Dimensions binned by node-centred cell, as a mask (bit d: dimension d),
+// from pic_binning::dual; -1 when the value is not one of 0, 1 and
+// PIC_DUAL_DIMS | mask.
+inline int dual_mask(int dual)
+{
+    if (dual == 0) return 0;
+    if (dual == 1) return 7;
+    if ((dual & ~7) == PIC_DUAL_DIMS) return dual & 7;
+    return -1;
+}
+
 // Bin coordinate floor((x - low)/h + dual/2) as a double (see pic_binning).
 __device__ __forceinline__ double bin_coord(double x, double low, double rdx, int dual)
 {

@@ -445,7 +445,7 @@ extern "C" int64_t pic_bin_count(const pic_binning* g)
         nb *= (g->ncell[d] + g->brick[d] - 1) / g->brick[d];
         bvol *= g->brick[d];
     }
-    if (g->dual != 0 && g->dual != 1) return PIC_ERR_ARG;
+    if (dual_mask(g->dual) < 0) return PIC_ERR_ARG;
     if (bvol > SORT_MAX_BVOL || nb * bvol >= int64_t(UINT32_MAX) || nb >= int64_t(INT32_MAX))
         return PIC_ERR_CAPACITY;
     return nb;
@@ -466,7 +466,7 @@ KeyGeom make_key_geom(const pic_mesh& mesh, const pic_binning& b)
         g.bvol *= b.brick[d];
     }
     g.rdx = 1.0 / mesh.h;
-    g.dual = b.dual;
+    g.dual = dual_mask(b.dual);
     g.vshift = 0;
     for (int d = 0; d < 3; ++d) {
         const int e = b.brick[d];

@@ -13,7 +13,7 @@ struct KeyGeom {
     int32_t brick[3];
     int32_t nb[3];
     uint32_t bvol;
-    int32_t dual;
+    int32_t dual;  // bit d: dimension d bins by node-centred cell
     // log2 of the brick edges and volume when all are powers of two (the
     // usual case), so the brick / bin split is shifts and masks; else -1.
     int32_t bshift[3];
@@ -33,7 +33,7 @@ __device__ __forceinline__ uint32_t cell_key(const KeyGeom& g, const PField<T>& 
     bool ok = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double c = bin_coord((double)x.at(xb, d), g.low[d], g.rdx, g.dual) -
+        const double c = bin_coord((double)x.at(xb, d), g.low[d], g.rdx, (g.dual >> d) & 1) -
                          (double)g.cell_lo[d];
         ok = ok && (c >= 0.0) && (c < (double)g.ncell[d]);
         rel[d] = ok ? (int32_t)c : 0;

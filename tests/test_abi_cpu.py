@@ -52,6 +52,10 @@ def test_host_only_entry_points():
     assert count((10, 10, 10), (4, 4, 4)) == 27
     assert count((0, 10, 10), (4, 4, 4)) == _abi.PIC_ERR_ARG
     assert count((10, 10, 10), (4, 4, 4), 2) == _abi.PIC_ERR_ARG
+    assert count((10, 10, 10), (4, 4, 4), 0x10 | 5) == 27
+    assert count((10, 10, 10), (4, 4, 4), 0x10) == 27
+    assert count((10, 10, 10), (4, 4, 4), 0x18) == _abi.PIC_ERR_ARG
+    assert count((10, 10, 10), (4, 4, 4), 0x20 | 1) == _abi.PIC_ERR_ARG
     assert count((64, 64, 64), (32, 32, 8)) == _abi.PIC_ERR_CAPACITY
 
 

@@ -145,24 +145,27 @@ def test_apic_entities_random(pa, entity, order):
 
 
 def _binning(pa, wl, order=2, entity=0, brick=(8, 8, 8)):
-    """Bins aligned with the spline's knot base for (order, entity)."""
-    dual = 1 if (order == 2) == (entity == 0) else 0
+    """Bins aligned with the spline's knot base for (order, entity): one more
+    bin in every dimension binned by node-centred cell."""
+    dual = pa.aligned_dual(order, entity)
+    flags = pa.dual_dims(dual)
     cell_lo, ncell = wl.cell_region
-    return pa.Binning(tuple(int(v) for v in cell_lo), tuple(int(v) + dual for v in ncell), brick,
-                      dual)
+    return pa.Binning(tuple(int(v) for v in cell_lo),
+                      tuple(int(v) + f for v, f in zip(ncell, flags)), brick, dual)
 
 
 @pytest.mark.parametrize("brick", [(1, 1, 1), (4, 4, 4), (8, 8, 8), (2, 4, 8)])
 @pytest.mark.parametrize("dt", ["f64", "f32"])
-@pytest.mark.parametrize("dual", [0, 1])
+@pytest.mark.parametrize("dual", [0, 1, 0x10 | 1, 0x10 | 6])
 def test_bin_sort_bit_exact(pa, wl0, brick, dt, dual):
     import gpu_util as gu
     tdt = torch.float64 if dt == "f64" else torch.float32
     xn = wl0.x if dt == "f64" else wl0.x.astype(np.float32)
     x = gu.soa(xn, tdt)
     cell_lo, ncell = wl0.cell_region
-    geom = pa.Binning(tuple(int(v) for v in cell_lo), tuple(int(v) + dual for v in ncell), brick,
-                      dual)
+    flags = pa.dual_dims(dual)
+    geom = pa.Binning(tuple(int(v) for v in cell_lo),
+                      tuple(int(v) + f for v, f in zip(ncell, flags)), brick, dual)
     res = pa.bin_sort(gu.ctx(), pa.Mesh(wl0.low, wl0.h), x, geom)
     gu.ctx().synchronize()
     keys, _ = orpic.cell_keys(xn.astype(np.float64), wl0.low, wl0.h, geom.cell_lo, geom.ncell,
@@ -292,6 +295,69 @@ def wl_dense():
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_brick_dense_cluster(pa, wl_dense, order, dtype):
     _check_brick(pa, wl_dense, order, 0, dtype, 4)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("entity", range(2, 8))
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_brick_staggered_vs_oracle(pa, wl0, order, entity, dtype):
+    """Face/Edge lattices binned by aligned_dual(order, entity), a dual flag
+    per dimension, take the brick kernels: p2g fills the scalar mass and
+    momentum grids of the staggered component, g2p (plain, and with the
+    normalisation fused in) rebuilds that column of c_p and zeroes the
+    others (Picasso_APIC.hpp:148-232, 477-544), all against the oracle."""
+    import gpu_util as gu
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    f32 = dtype == "f32"
+    rnd = (lambda a: a.astype(np.float32).astype(np.float64)) if f32 else (lambda a: a)
+    xh, mh, ch = rnd(wl0.x), rnd(wl0.m), rnd(wl0.c)
+    lo, n = wl0.node_window
+    shp = tuple(int(v) for v in n)
+    mesh = pa.Mesh(wl0.low, wl0.h)
+    x = gu.soa(xh, tdt)
+    m = gu.soa(mh[:, None], tdt)
+    c = gu.soa(ch.reshape(-1, 12), tdt)
+    ctx = gu.ctx()
+    bins = pa.bin_sort(ctx, mesh, x, _binning(pa, wl0, order, entity, (4, 4, 4)))
+    sd = pa.create_spline(entity, order, mesh, x, bins)
+    gm = pa.GridField(torch.zeros(shp, dtype=tdt, device=gu.DEV), tuple(int(v) for v in lo))
+    gmu = pa.GridField(torch.zeros(shp, dtype=tdt, device=gu.DEV), tuple(int(v) for v in lo))
+    gfield = rnd(np.random.default_rng(entity).normal(size=shp))
+    cg = torch.zeros_like(c)
+    cn = torch.zeros_like(c)
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    try:
+        pa.APIC.p2g(ctx, m, c, gm, gmu, sd, zero_first=True)
+        pa.APIC.g2p(ctx, pa.GridField(gu.dev(gfield, tdt), tuple(int(v) for v in lo)), cg, sd)
+        pa.APIC.g2p_normalized(ctx, gm, gmu, cn, sd)
+        ctx.synchronize()
+        # The brick path ends with its stale-list pass, which the
+        # per-particle kernels do not have.
+        brick_runs = [ctx.kernel_time(k)[1] for k in ("apic_p2g_stale", "apic_g2p_stale")]
+    finally:
+        ctx.set_timing(False)
+    assert brick_runs[0] == 1 and brick_runs[1] == 2
+    rm = np.zeros(shp)
+    rmu = np.zeros(shp)
+    orpic.apic_p2g(order, entity, wl0.low, wl0.h, lo, n, xh, mh, ch, rm, rmu)
+    rc = orpic.apic_g2p(order, entity, wl0.low, wl0.h, lo, n, xh, gfield)
+    gmh, gmuh = gu.host(gm.tensor), gu.host(gmu.tensor)
+    u = np.where(gmh > 0, gmuh / np.where(gmh > 0, gmh, 1.0), 0.0)
+    rn = orpic.apic_g2p(order, entity, wl0.low, wl0.h, lo, n, xh, u)
+    tol = F32_TOL if f32 else F64_TOL
+    assert gu.rel_inf(gmh, rm) < tol
+    assert gu.rel_inf(gmuh, rmu) < tol
+    cgh = gu.host(cg).reshape(-1, 4, 3)
+    cnh = gu.host(cn).reshape(-1, 4, 3)
+    dim = (entity - 2) % 3
+    others = [d for d in range(3) if d != dim]
+    assert np.all(cgh[:, :, others] == 0) and np.all(cnh[:, :, others] == 0)
+    assert gu.rel_inf(cgh[:, 0], rc[:, 0]) < tol
+    assert gu.rel_inf(cgh[:, 1:], rc[:, 1:]) < tol * 10
+    assert gu.rel_inf(cnh[:, 0], rn[:, 0]) < tol * 10
+    assert gu.rel_inf(cnh[:, 1:], rn[:, 1:]) < tol * 100
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])

@@ -209,6 +209,48 @@ def test_cell_keys_and_stable_sort(brick):
     np.testing.assert_array_equal(start, np.concatenate([[0], np.cumsum(hist)]))
 
 
+@pytest.mark.parametrize("mask", range(8))
+def test_cell_keys_dual_per_dimension(mask):
+    """A per-dimension dual (0x10 | mask) bins each dimension exactly as the
+    all-mesh (dual 0) or all-dual (dual 1) binning does in that dimension;
+    values outside 0, 1 and 0x10 | mask are rejected."""
+    rng = np.random.default_rng(40 + mask)
+    ncell = np.array([12, 10, 15], np.int32)
+    cell_lo = np.array([2, 3, 1], np.int32)
+    x = LOW + H * (cell_lo + 0.5 + rng.uniform(0, 1, size=(2000, 3)) * (ncell - 1))
+    _, c0 = orpic.cell_keys(x, LOW, H, cell_lo, ncell, (4, 4, 4), 0)
+    _, c1 = orpic.cell_keys(x, LOW, H, cell_lo, ncell, (4, 4, 4), 1)
+    _, cm = orpic.cell_keys(x, LOW, H, cell_lo, ncell, (4, 4, 4), 0x10 | mask)
+    for d in range(3):
+        np.testing.assert_array_equal(cm[:, d], (c1 if (mask >> d) & 1 else c0)[:, d])
+    # the dual cell of a point half a cell past a node is the next one
+    assert np.all(c1 - c0 >= 0) and np.all(c1 - c0 <= 1)
+    for bad in (2, 0x20, 0x18, -1):
+        with pytest.raises(orpic.OracleError):
+            orpic.cell_keys(x, LOW, H, cell_lo, ncell, (4, 4, 4), bad)
+
+
+def test_aligned_dual_matches_the_knot_base():
+    """aligned_dual(order, entity) bins by the spline's knot base in every
+    dimension: on its bins, the first stencil knot of every particle is the
+    same offset from its bin (checked through the oracle's spline), for all
+    orders and all eight entity lattices."""
+    import picasso_amd as pa
+    rng = np.random.default_rng(5)
+    x = LOW + H * (6 + rng.uniform(0, 8, size=(400, 3)))
+    for order in (1, 2, 3):
+        for entity in range(8):
+            dual = pa.aligned_dual(order, entity)
+            flags = pa.dual_dims(dual)
+            _, cells = orpic.cell_keys(x, LOW, H, [0, 0, 0], [40, 40, 40], (1, 1, 1), dual)
+            base = np.array([orpic.spline(order, entity, LOW, H, xp)["s"][:, 0] for xp in x])
+            off = base - cells
+            assert np.all(off == off[0]), (order, entity, dual)
+            if entity in (0, 1):  # the collocated rule the brick kernels always used
+                assert dual == (1 if (order == 2) == (entity == 0) else 0)
+            assert all(f in (0, 1) for f in flags)
+
+
 def test_cell_keys_out_of_range_is_error():
     x = np.array([[LOW[0] - 0.01, LOW[1], LOW[2]]])
     with pytest.raises(orpic.OracleError):
