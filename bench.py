@@ -345,6 +345,78 @@ def config_interp(pa, ctx, synth, steps=10):
             "kernels_ms": kt}
 
 
+def config_mac(pa, ctx, synth, steps=10, steps_particles=3):
+    """APIC on the three Face lattices of a MAC grid (Picasso_APIC.hpp:148-232,
+    477-544) on the 256^3 sphere, FP32, quadratic splines, particles stored
+    in node-bin order.  Per face: a sort on that face's knot-aligned bins (a
+    dual flag per dimension), p2g into the face's mass and momentum grids,
+    g2p with the normalisation fused into the face's own c_p (a staggered
+    g2p zeroes the other columns).  The same transfers through the
+    per-particle kernels (no bins) are timed beside it."""
+    import torch
+    wl = synth.apic_sphere(256, 8)
+    P = wl.np
+    dev = "cuda:0"
+    mesh = pa.Mesh(wl.low, wl.h)
+    x = torch.as_tensor(np.ascontiguousarray(wl.x.astype(np.float32).T), device=dev).t()
+    m = torch.as_tensor(wl.m.astype(np.float32)[None, :], device=dev).t()
+    c = torch.as_tensor(np.ascontiguousarray(wl.c.astype(np.float32).reshape(P, 12).T),
+                        device=dev).t()
+    cell_lo, ncell = wl.cell_region
+    node_geom = pa.Binning(tuple(int(v) for v in cell_lo), tuple(int(v) + 1 for v in ncell),
+                           BRICK, 1)
+    order = pa.bin_sort(ctx, mesh, x, node_geom).perm.long()
+    ctx.synchronize()
+    x = x.t()[:, order].contiguous().t()
+    m = m.t()[:, order].contiguous().t()
+    c = c.t()[:, order].contiguous().t()
+    lo, n = wl.node_window
+    shp, lot = tuple(int(k) for k in n), tuple(int(k) for k in lo)
+    faces = []
+    for d in range(3):
+        ent = pa.FieldLocation.Face(d)
+        dual = pa.aligned_dual(2, ent)
+        geom = pa.Binning(tuple(int(v) for v in cell_lo),
+                          tuple(int(v) + f for v, f in zip(ncell, pa.dual_dims(dual))), BRICK,
+                          dual)
+        bins = pa.bin_sort(ctx, mesh, x, geom)
+        faces.append(dict(geom=geom, bins=bins, sd=pa.create_spline(ent, 2, mesh, x, bins),
+                          sd_pp=pa.create_spline(ent, 2, mesh, x),
+                          gm=pa.GridField(torch.zeros(shp, device=dev), lot),
+                          gmu=pa.GridField(torch.zeros(shp, device=dev), lot),
+                          cf=torch.zeros((12, P), device=dev).t()))
+
+    def step_bricks():
+        for f in faces:
+            pa.bin_sort(ctx, mesh, x, f["geom"], out=f["bins"])
+            pa.APIC.p2g(ctx, m, c, f["gm"], f["gmu"], f["sd"], zero_first=True)
+            pa.APIC.g2p_normalized(ctx, f["gm"], f["gmu"], f["cf"], f["sd"])
+
+    def step_particles():
+        for f in faces:
+            pa.APIC.p2g(ctx, m, c, f["gm"], f["gmu"], f["sd_pp"], zero_first=True)
+            pa.APIC.g2p_normalized(ctx, f["gm"], f["gmu"], f["cf"], f["sd_pp"])
+
+    fam = ["bin_sort", "zero_grid", "apic_p2g", "apic_g2p"]
+    out = {"workload": "APIC MAC grid: the three Face lattices on the APIC sphere 256^3 x 8 ppc, "
+                       "quadratic splines, FP32, per face sort -> p2g -> g2p (normalisation "
+                       "fused), particles in node-bin order",
+           "particles": P}
+    for name, fn, k in (("bricks", step_bricks, steps), ("per_particle", step_particles,
+                                                           steps_particles)):
+        fn()
+        ctx.synchronize()
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        ms = _time(fn, k, 0)
+        ctx.set_timing(False)
+        ctx.synchronize()
+        out[name] = {"ms_per_step": ms, "particles_per_s": P / (ms * 1e-3),
+                     "kernels_ms": _kernel_avgs(ctx, fam)}
+        ctx.reset_timing()
+    return out
+
+
 def config_batched(pa, ctx, synth, n_inst=512, steps=3, lanes=4):
     """Independent instances: grid edge n_k = round(16 * 4^u_k), u_k ~ U(0, 1)
     (16^3 to 64^3 cells, log-uniform), ppc_k ~ U{2..16}, fields as the 32^3
@@ -626,6 +698,10 @@ def main():
         log("config: generic transfers 256^3 FP32 ...")
         others["generic_interp"] = config_interp(pa, ctx, synth)
         log(f"  {others['generic_interp']}")
+        torch.cuda.empty_cache()
+        log("config: MAC grid, three Face lattices, 256^3 FP32 ...")
+        others["apic_mac_faces"] = config_mac(pa, ctx, synth)
+        log(f"  {others['apic_mac_faces']}")
         torch.cuda.empty_cache()
         log("config: 512 batched instances ...")
         others["batched_instances"] = config_batched(pa, ctx, synth)

@@ -30,9 +30,13 @@
  *    bin's contributions in FP32 registers, adds the bin sums that land on a
  *    node in FP64, and applies one FP32 atomic per node and brick.
  *  - Brick kernels: p2g and g2p take the brick path when bins are given, the
- *    binning is aligned with the spline's knot base (dual = 1 for quadratic
- *    Node or linear/cubic Cell, dual = 0 for linear/cubic Node or quadratic
- *    Cell) and bricks are 4x4x4 (p2g) or 4x4x4 / 8x8x8 (g2p).  Otherwise the
+ *    binning is aligned with the spline's knot base in every dimension
+ *    (dual_d = 1 where the spline is quadratic on a node-aligned dimension
+ *    or linear/cubic on a cell-centred one, else 0: dual = 1 for quadratic
+ *    Node or linear/cubic Cell, 0 for linear/cubic Node or quadratic Cell,
+ *    PIC_DUAL_DIMS | mask for the staggered Face/Edge lattices) and bricks
+ *    are 4x4x4 (p2g) or 4x4x4 / 8x8x8 (g2p).  APIC takes them on every
+ *    entity lattice, PolyPIC on collocated ones.  Otherwise the
  *    per-particle atomic scatter / gather kernels run.  Particles whose
  *    position no longer matches their bin go through the per-particle path,
  *    so results never depend on the bins being current.
