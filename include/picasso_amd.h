@@ -35,8 +35,9 @@
  *    or linear/cubic on a cell-centred one, else 0: dual = 1 for quadratic
  *    Node or linear/cubic Cell, 0 for linear/cubic Node or quadratic Cell,
  *    PIC_DUAL_DIMS | mask for the staggered Face/Edge lattices) and bricks
- *    are 4x4x4 (p2g) or 4x4x4 / 8x8x8 (g2p).  APIC takes them on every
- *    entity lattice, PolyPIC on collocated ones.  Otherwise the
+ *    are 4x4x4 (p2g) or 4x4x4 / 8x8x8 (g2p; PolyPIC 4x4x4).  APIC and
+ *    PolyPIC take them on every entity lattice (PolyPIC collocated with
+ *    three columns).  Otherwise the
  *    per-particle atomic scatter / gather kernels run.  Particles whose
  *    position no longer matches their bin go through the per-particle path,
  *    so results never depend on the bins being current.

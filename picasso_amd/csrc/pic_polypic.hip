@@ -274,7 +274,7 @@ using PolyStage = BrickStage<(PolyBrickShape<ORDER, T>::CAP + PolyBrickShape<ORD
                                  PolyBrickShape<ORDER, T>::NT,
                              PolyBrickShape<ORDER, T>::NT, PolyBrickShape<ORDER, T>::BINS>;
 
-template <int ORDER, typename T>
+template <int ORDER, typename T, bool STAG>
 __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
                                                       const BrickItem& it,
                                                       BrickWalk<true, PolyBrickArgs<T>>& w,
@@ -377,7 +377,9 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
             const int64_t cb = a.c.base(int64_t(pidx[q]));
 #pragma unroll
             for (int r = 0; r < NM; ++r) {
-                const T cr = a.c.at(cb, r * 3 + d);
+                // Staggered: every momentum work-item takes the one column
+                // the transfer works on; only its sum is written.
+                const T cr = STAG ? a.c.at(cb, r * a.pcols + a.col) : a.c.at(cb, r * 3 + d);
                 C[MODE_EXP[r][0]][MODE_EXP[r][1]][MODE_EXP[r][2]] =
                     comp == 0 ? (r == 0 ? T(1) : T(0)) : cr;
             }
@@ -422,17 +424,19 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
         if (r == 0) w.publish();
         __syncthreads();
         if (r == 0) st.prefetch_next(g, a.perm, w, CAP);
-        gather_flush<NK, PB, NT, S::CPR>(lds, f0, r_lo, 1, a.gm, a.gmu, a.err);
+        gather_flush<NK, PB, NT, S::CPR>(lds, f0, r_lo, 1, a.gm, a.gmu, a.err, STAG ? 2 : 4);
         if (r + 1 < S::NROUND) __syncthreads();
     }
 }
 
-// Brick g2p (collocated, three components): the brick's (4+ORDER)^3 grid
-// footprint is staged in LDS once, then one work-item per particle forms
-// its modes, the Taylor coefficients sum_n F0_e0(i) F1_e1(j) F2_e2(k) u_n
-// (F_e = weight, gradient, half second derivative), summed one dimension
-// at a time (k, then j, then i) instead of per node and mode.
-template <int ORDER, typename T>
+// Brick g2p (three components, or the one of a staggered field): the
+// brick's (4+ORDER)^3 grid footprint is staged in LDS once, then one
+// work-item per particle forms its modes, the Taylor coefficients
+// sum_n F0_e0(i) F1_e1(j) F2_e2(k) u_n (F_e = weight, gradient, half second
+// derivative), summed one dimension at a time (k, then j, then i) instead
+// of per node and mode.  A staggered g2p rewrites only its column
+// (Picasso_PolyPIC.hpp:328).
+template <int ORDER, typename T, bool STAG>
 __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
                                                       const BrickItem& it,
                                                       BrickWalk<false, PolyBrickArgs<T>>& w,
@@ -444,6 +448,7 @@ __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
     constexpr int FV = FN * FN * FN;
     constexpr int NM = Modes<ORDER>::NM;
     constexpr int E = ORDER == 1 ? 2 : 3;
+    constexpr int NC = STAG ? 1 : 3;
     const PolyArgs<T>& a = g.a;
     const uint32_t beg = it.beg, end = it.end;
     int f0[3];
@@ -455,7 +460,7 @@ __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
         const bool in = gu.covers(0, ni, ni) && gu.covers(1, nj, nj) && gu.covers(2, nk, nk);
         const int64_t off = in ? gu.off(ni, nj, nk) : 0;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) us[d * FV + q] = in ? gu.data[off + d * gu.cs] : T(0);
+        for (int d = 0; d < NC; ++d) us[d * FV + q] = in ? gu.data[off + d * gu.cs] : T(0);
     }
     w.publish();
     __syncthreads();
@@ -481,7 +486,7 @@ __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
                 if constexpr (E > 2) F[q][2][n] = T(0.5) * sd.g2[q][n];
             }
         const int lbase = (li * FN + lj) * FN + lk;
-        T cm[E][E][E][3];
+        T cm[E][E][E][NC];
 #pragma unroll
         for (int e0 = 0; e0 < E; ++e0)
 #pragma unroll
@@ -489,38 +494,40 @@ __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
 #pragma unroll
                 for (int e2 = 0; e2 < E; ++e2)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) cm[e0][e1][e2][d] = T(0);
+                    for (int d = 0; d < NC; ++d) cm[e0][e1][e2][d] = T(0);
 #pragma unroll
         for (int i = 0; i < NK; ++i) {
-            T t[E][E][3];
+            T t[E][E][NC];
 #pragma unroll
             for (int e1 = 0; e1 < E; ++e1)
 #pragma unroll
                 for (int e2 = 0; e2 < E; ++e2)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) t[e1][e2][d] = T(0);
+                    for (int d = 0; d < NC; ++d) t[e1][e2][d] = T(0);
 #pragma unroll
             for (int j = 0; j < NK; ++j) {
-                T sk[E][3];
+                T sk[E][NC];
 #pragma unroll
                 for (int e2 = 0; e2 < E; ++e2)
 #pragma unroll
-                    for (int d = 0; d < 3; ++d) sk[e2][d] = T(0);
+                    for (int d = 0; d < NC; ++d) sk[e2][d] = T(0);
 #pragma unroll
                 for (int k = 0; k < NK; ++k) {
                     const int q = lbase + (i * FN + j) * FN + k;
-                    const T ue[3] = {us[q], us[FV + q], us[2 * FV + q]};
+                    T ue[NC];
+#pragma unroll
+                    for (int d = 0; d < NC; ++d) ue[d] = us[d * FV + q];
 #pragma unroll
                     for (int e2 = 0; e2 < E; ++e2)
 #pragma unroll
-                        for (int d = 0; d < 3; ++d) sk[e2][d] += F[2][e2][k] * ue[d];
+                        for (int d = 0; d < NC; ++d) sk[e2][d] += F[2][e2][k] * ue[d];
                 }
 #pragma unroll
                 for (int e1 = 0; e1 < E; ++e1)
 #pragma unroll
                     for (int e2 = 0; e2 < E; ++e2)
 #pragma unroll
-                        for (int d = 0; d < 3; ++d) t[e1][e2][d] += F[1][e1][j] * sk[e2][d];
+                        for (int d = 0; d < NC; ++d) t[e1][e2][d] += F[1][e1][j] * sk[e2][d];
             }
 #pragma unroll
             for (int e0 = 0; e0 < E; ++e0)
@@ -529,29 +536,34 @@ __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
 #pragma unroll
                     for (int e2 = 0; e2 < E; ++e2)
 #pragma unroll
-                        for (int d = 0; d < 3; ++d) cm[e0][e1][e2][d] += F[0][e0][i] * t[e1][e2][d];
+                        for (int d = 0; d < NC; ++d) cm[e0][e1][e2][d] += F[0][e0][i] * t[e1][e2][d];
         }
         const int64_t cb = a.c.base(p);
 #pragma unroll
-        for (int r = 0; r < NM; ++r)
+        for (int r = 0; r < NM; ++r) {
+            const auto& ce = cm[MODE_EXP[r][0]][MODE_EXP[r][1]][MODE_EXP[r][2]];
+            if constexpr (STAG) {
+                a.c.at(cb, r * a.pcols + a.col) = ce[0];
+            } else {
 #pragma unroll
-            for (int d = 0; d < 3; ++d)
-                a.c.at(cb, r * 3 + d) = cm[MODE_EXP[r][0]][MODE_EXP[r][1]][MODE_EXP[r][2]][d];
+                for (int d = 0; d < 3; ++d) a.c.at(cb, r * 3 + d) = ce[d];
+            }
+        }
     }
 }
 
-template <int ORDER, typename T>
+template <int ORDER, typename T, bool STAG>
 __global__ __launch_bounds__(256) void k_polypic_g2p_brick(PolyBrickArgs<T> g)
 {
     constexpr int FN = 4 + ORDER;
-    __shared__ T us[3 * FN * FN * FN];
+    __shared__ T us[(STAG ? 1 : 3) * FN * FN * FN];
     __shared__ BrickItem q[2];
     walk_bricks<false>(g, q, [&](const BrickItem& it, BrickWalk<false, PolyBrickArgs<T>>& w) {
-        polypic_g2p_one_brick<ORDER, T>(g, it, w, us);
+        polypic_g2p_one_brick<ORDER, T, STAG>(g, it, w, us);
     });
 }
 
-template <int ORDER, typename T>
+template <int ORDER, typename T, bool STAG>
 __global__ __launch_bounds__((PolyBrickShape<ORDER, T>::NT), (PolyBrickShape<ORDER, T>::WAVES)) void k_polypic_p2g_brick(
     PolyBrickArgs<T> g)
 {
@@ -563,15 +575,16 @@ __global__ __launch_bounds__((PolyBrickShape<ORDER, T>::NT), (PolyBrickShape<ORD
     for (int i = threadIdx.x; i < 4 * S::BINS; i += S::NT) rng[i] = 0u;
     PolyStage<ORDER, T> st{rng, 0, false, {}};
     walk_bricks<true>(g, q, [&](const BrickItem& it, BrickWalk<true, PolyBrickArgs<T>>& w) {
-        polypic_p2g_one_brick<ORDER, T>(g, it, w, lds, pidx, st);
+        polypic_p2g_one_brick<ORDER, T, STAG>(g, it, w, lds, pidx, st);
     });
 }
 
 template <typename T>
-static int polypic_p2g_brick_run(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
-                                 int64_t np, const pic_pfield* x, const pic_pfield* m,
-                                 const pic_pfield* u, const pic_pfield* c, const pic_grid& gmu,
-                                 const pic_grid* gm, double dt, const pic_bins& bins);
+static int polypic_p2g_brick_run(pic_context* ctx, int order, int entity, int ncomp,
+                                 const pic_mesh& mesh, int64_t np, const pic_pfield* x,
+                                 const pic_pfield* m, const pic_pfield* u, const pic_pfield* c,
+                                 const pic_grid& gmu, const pic_grid* gm, double dt,
+                                 const pic_bins& bins);
 
 template <typename T>
 struct PolyLaunch {
@@ -628,60 +641,77 @@ static PolyArgs<T> make_poly_args(pic_context* ctx, int entity, int ncomp, const
     return a;
 }
 
+// Brick paths for a three-column collocated field or any staggered one
+// (ncomp is the particle field's column count).
 template <typename T>
-static int polypic_p2g_brick_run(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
-                                 int64_t np, const pic_pfield* x, const pic_pfield* m,
-                                 const pic_pfield* u, const pic_pfield* c, const pic_grid& gmu,
-                                 const pic_grid* gm, double dt, const pic_bins& bins)
+static int polypic_p2g_brick_run(pic_context* ctx, int order, int entity, int ncomp,
+                                 const pic_mesh& mesh, int64_t np, const pic_pfield* x,
+                                 const pic_pfield* m, const pic_pfield* u, const pic_pfield* c,
+                                 const pic_grid& gmu, const pic_grid* gm, double dt,
+                                 const pic_bins& bins)
 {
     if (brick_edge(bins.geom) != 4) return PIC_ERR_UNSUPPORTED;
     PolyBrickArgs<T> g;
     if (!setup_walk(ctx, order, entity, np, bins, g)) return PIC_ERR_UNSUPPORTED;
-    g.a = make_poly_args<T>(ctx, entity, 3, mesh, np, x, m, u, c, gmu, gm, dt, bins.perm);
+    g.a = make_poly_args<T>(ctx, entity, ncomp, mesh, np, x, m, u, c, gmu, gm, dt, bins.perm);
+    const bool stag = stag_dim(entity) >= 0;
     {
         TimedLaunch tl(ctx, "polypic_p2g");
         auto go = [&](auto kernel, int nt) {
             const unsigned nbk = persistent_blocks(ctx, kernel, nt, 0, g.nbricks);
             kernel<<<nbk, nt, 0, ctx->stream>>>(g);
         };
-        if (order == 1) go(k_polypic_p2g_brick<1, T>, PolyBrickShape<1, T>::NT);
-        else if (order == 2) go(k_polypic_p2g_brick<2, T>, PolyBrickShape<2, T>::NT);
-        else go(k_polypic_p2g_brick<3, T>, PolyBrickShape<3, T>::NT);
+        if (stag) {
+            if (order == 1) go(k_polypic_p2g_brick<1, T, true>, PolyBrickShape<1, T>::NT);
+            else if (order == 2) go(k_polypic_p2g_brick<2, T, true>, PolyBrickShape<2, T>::NT);
+            else go(k_polypic_p2g_brick<3, T, true>, PolyBrickShape<3, T>::NT);
+        } else {
+            if (order == 1) go(k_polypic_p2g_brick<1, T, false>, PolyBrickShape<1, T>::NT);
+            else if (order == 2) go(k_polypic_p2g_brick<2, T, false>, PolyBrickShape<2, T>::NT);
+            else go(k_polypic_p2g_brick<3, T, false>, PolyBrickShape<3, T>::NT);
+        }
         if (launch_status()) return PIC_ERR_HIP;
     }
     // Particles the brick kernel skipped (stale bins).
     TimedLaunch tl(ctx, "polypic_p2g_stale");
     PolyLaunch<T> L{g.a, ctx->stream, true};
     L.a.list = g.stale;
-    L.run(order, 3, false);
+    L.run(order, ncomp, stag);
     return launch_status();
 }
 
 template <typename T>
-static int polypic_g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
-                                 int64_t np, const pic_pfield* x, const pic_pfield* c,
-                                 const pic_grid& gu, const pic_bins& bins)
+static int polypic_g2p_brick_run(pic_context* ctx, int order, int entity, int ncomp,
+                                 const pic_mesh& mesh, int64_t np, const pic_pfield* x,
+                                 const pic_pfield* c, const pic_grid& gu, const pic_bins& bins)
 {
     if (brick_edge(bins.geom) != 4) return PIC_ERR_UNSUPPORTED;
     PolyBrickArgs<T> g;
     if (!setup_walk(ctx, order, entity, np, bins, g)) return PIC_ERR_UNSUPPORTED;
-    g.a = make_poly_args<T>(ctx, entity, 3, mesh, np, x, nullptr, nullptr, c, gu, nullptr, 0.0,
-                            bins.perm);
+    g.a = make_poly_args<T>(ctx, entity, ncomp, mesh, np, x, nullptr, nullptr, c, gu, nullptr,
+                            0.0, bins.perm);
+    const bool stag = stag_dim(entity) >= 0;
     {
         TimedLaunch tl(ctx, "polypic_g2p");
         auto go = [&](auto kernel) {
             const unsigned nbk = persistent_blocks(ctx, kernel, 256, 0, g.nbricks);
             kernel<<<nbk, 256, 0, ctx->stream>>>(g);
         };
-        if (order == 1) go(k_polypic_g2p_brick<1, T>);
-        else if (order == 2) go(k_polypic_g2p_brick<2, T>);
-        else go(k_polypic_g2p_brick<3, T>);
+        if (stag) {
+            if (order == 1) go(k_polypic_g2p_brick<1, T, true>);
+            else if (order == 2) go(k_polypic_g2p_brick<2, T, true>);
+            else go(k_polypic_g2p_brick<3, T, true>);
+        } else {
+            if (order == 1) go(k_polypic_g2p_brick<1, T, false>);
+            else if (order == 2) go(k_polypic_g2p_brick<2, T, false>);
+            else go(k_polypic_g2p_brick<3, T, false>);
+        }
         if (launch_status()) return PIC_ERR_HIP;
     }
     TimedLaunch tl(ctx, "polypic_g2p_stale");
     PolyLaunch<T> L{g.a, ctx->stream, false};
     L.a.list = g.stale;
-    L.run(order, 3, false);
+    L.run(order, ncomp, stag);
     return launch_status();
 }
 
@@ -723,12 +753,12 @@ extern "C" int pic_polypic_p2g(pic_context* ctx, int order, int entity, int dtyp
         if (rc) return rc;
     }
     if (np == 0) return PIC_OK;
-    if (bins && !stag && ncomp == 3) {
+    if (bins && (stag || ncomp == 3)) {
         const int rc = dtype == PIC_F64
-                           ? polypic_p2g_brick_run<double>(ctx, order, entity, *mesh, np, x, m_p,
-                                                           u_p, c_p, *mu_i, m_i, dt, *bins)
-                           : polypic_p2g_brick_run<float>(ctx, order, entity, *mesh, np, x, m_p,
-                                                          u_p, c_p, *mu_i, m_i, dt, *bins);
+                           ? polypic_p2g_brick_run<double>(ctx, order, entity, ncomp, *mesh, np, x,
+                                                           m_p, u_p, c_p, *mu_i, m_i, dt, *bins)
+                           : polypic_p2g_brick_run<float>(ctx, order, entity, ncomp, *mesh, np, x,
+                                                          m_p, u_p, c_p, *mu_i, m_i, dt, *bins);
         if (rc != PIC_ERR_UNSUPPORTED) return rc;
     }
     const uint32_t* perm = bins ? bins->perm : nullptr;
@@ -762,12 +792,12 @@ extern "C" int pic_polypic_g2p(pic_context* ctx, int order, int entity, int dtyp
     DeviceGuard g(ctx->device);
     if (!g.ok) return PIC_ERR_HIP;
     const bool stag = stag_dim(entity) >= 0;
-    if (bins && !stag && ncomp == 3) {
+    if (bins && (stag || ncomp == 3)) {
         const int rc = dtype == PIC_F64
-                           ? polypic_g2p_brick_run<double>(ctx, order, entity, *mesh, np, x, c_p,
-                                                           *u_i, *bins)
-                           : polypic_g2p_brick_run<float>(ctx, order, entity, *mesh, np, x, c_p,
-                                                          *u_i, *bins);
+                           ? polypic_g2p_brick_run<double>(ctx, order, entity, ncomp, *mesh, np, x,
+                                                           c_p, *u_i, *bins)
+                           : polypic_g2p_brick_run<float>(ctx, order, entity, ncomp, *mesh, np, x,
+                                                          c_p, *u_i, *bins);
         if (rc != PIC_ERR_UNSUPPORTED) return rc;
     }
     const uint32_t* perm = bins ? bins->perm : nullptr;

@@ -630,6 +630,69 @@ def test_g2p_update_F_matches_oracle(pa, wl0, order, dtype, binned, fused_norm):
     assert gu.rel_inf(gu.host(Finv).reshape(-1, 3, 3), rFinv) < ftol
 
 
+@pytest.mark.parametrize("order", [1, 2, 3])
+@pytest.mark.parametrize("entity", [2, 4, 6])
+@pytest.mark.parametrize("ncomp", [1, 3])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_polypic_brick_staggered_vs_oracle(pa, wl0, order, entity, ncomp, dtype):
+    """Staggered PolyPIC (Picasso_PolyPIC.hpp:138-229, 299-359) on bins with a
+    dual flag per dimension takes the brick kernels: p2g projects column
+    `dim` (entity dim for a 3-column field, else 0) onto the scalar grid,
+    g2p rewrites only that column; against the oracle."""
+    import gpu_util as gu
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    f32 = dtype == "f32"
+    rng = np.random.default_rng(300 + 10 * order + entity + ncomp)
+    P = len(wl0.x)
+    nm = orpic.polypic_nmode(order)
+    x, m = wl0.x.copy(), wl0.m.copy()
+    u = rng.normal(0, 0.5, size=(P, nm, 3))
+    cm = rng.normal(size=(P, nm, ncomp))
+    c0 = rng.normal(size=(P, nm, ncomp))
+    if f32:
+        x, m, u, cm, c0 = (a.astype(np.float32).astype(np.float64) for a in (x, m, u, cm, c0))
+    lo, n = wl0.node_window
+    shp = tuple(int(v) for v in n)
+    mesh = pa.Mesh(wl0.low, wl0.h)
+    xt = gu.soa(x, tdt)
+    ctx = gu.ctx()
+    bins = pa.bin_sort(ctx, mesh, xt, _binning(pa, wl0, order, entity, (4, 4, 4)))
+    sd = pa.create_spline(entity, order, mesh, xt, bins)
+    dt = 1e-3
+    gmu = pa.GridField(gu.dev(np.zeros(shp), tdt))
+    gm = pa.GridField(gu.dev(np.zeros(shp), tdt))
+    gfield = rng.normal(size=shp)
+    if f32:
+        gfield = gfield.astype(np.float32).astype(np.float64)
+    cg = gu.soa(c0.reshape(P, -1), tdt)
+    ctx.synchronize()
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    try:
+        pa.PolyPIC.p2g(ctx, gu.soa(m[:, None], tdt), gu.soa(u.reshape(P, -1), tdt),
+                       gu.soa(cm.reshape(P, -1), tdt), gmu, gm, dt, sd, ncomp=ncomp,
+                       zero_first=True)
+        pa.PolyPIC.g2p(ctx, pa.GridField(gu.dev(gfield, tdt)), cg, sd, ncomp=ncomp)
+        ctx.synchronize()
+        runs = [ctx.kernel_time(k)[1] for k in ("polypic_p2g_stale", "polypic_g2p_stale")]
+    finally:
+        ctx.set_timing(False)
+    assert runs == [1, 1]  # the brick path ran (its stale-list pass is timed)
+    rmu = np.zeros(shp)
+    rm = np.zeros(shp)
+    orpic.polypic_p2g(order, entity, wl0.low, wl0.h, lo, n, x, m, u, cm, dt, rmu, rm)
+    rc = orpic.polypic_g2p(order, entity, wl0.low, wl0.h, lo, n, x, gfield, ncomp, c=c0.copy())
+    tol = F32_TOL if f32 else F64_TOL
+    assert gu.rel_inf(gu.host(gm.tensor), rm) < tol
+    assert gu.rel_inf(gu.host(gmu.tensor), rmu) < tol
+    cgh = gu.host(cg).reshape(P, nm, ncomp)
+    col = (entity - 2) % 3 if ncomp == 3 else 0
+    others = [d for d in range(ncomp) if d != col]
+    np.testing.assert_array_equal(cgh[:, :, others], c0[:, :, others].astype(cgh.dtype))
+    for lo_r, hi_r in ((0, 1), (1, 4), (4, nm)):
+        assert gu.rel_inf(cgh[:, lo_r:hi_r, col], rc[:, lo_r:hi_r, col]) < tol * 10
+
+
 def _polypic_binned_check(pa, wl, order, dtype, stale=False):
     """PolyPIC p2g through the brick kernel (bins from pic_bin_sort) against
     the oracle, three grid components."""
