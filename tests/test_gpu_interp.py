@@ -151,8 +151,7 @@ def _aligned_spline(pa, gu, entity, order, x, tdt, x_bins=None):
     entity), so P2G runs on the brick kernel; bins are sorted from x_bins
     (default x), so moved particles exercise the stale list."""
     mesh = pa.Mesh(LOW, H)
-    dual = 1 if (order == 2) == (entity == 0) else 0
-    geom = pa.Binning((3, 3, 3), (12, 12, 12), (4, 4, 4), dual)
+    geom = pa.Binning((3, 3, 3), (12, 12, 12), (4, 4, 4), pa.aligned_dual(order, entity))
     xb = gu.soa(x if x_bins is None else x_bins, tdt)
     bins = pa.bin_sort(gu.ctx(), mesh, xb, geom)
     return pa.create_spline(entity, order, mesh, gu.soa(x, tdt), bins)
@@ -160,11 +159,12 @@ def _aligned_spline(pa, gu, entity, order, x, tdt, x_bins=None):
 
 @pytest.mark.parametrize("moved", [False, True])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-@pytest.mark.parametrize("entity", [0, 1])
+@pytest.mark.parametrize("entity", [0, 1, 3, 7])
 @pytest.mark.parametrize("order", [1, 2, 3])
 def test_p2g_interp_brick_vs_oracle(pa, order, entity, dtype, moved):
     """Binned P2G on the brick kernel (and, with moved particles, its stale
-    list) against the oracle."""
+    list) against the oracle, on node, cell and staggered (Face<J>,
+    Edge<K>: bins with a dual flag per dimension) lattices."""
     import gpu_util as gu
     tdt = torch.float64 if dtype == "f64" else torch.float32
     tol = 1e-12 if dtype == "f64" else 1e-5
@@ -184,11 +184,20 @@ def test_p2g_interp_brick_vs_oracle(pa, order, entity, dtype, moved):
         field = pa.GridField(torch.zeros(shape + (_p2g_ngrid(op, nc),), dtype=tdt,
                                          device=gu.DEV), tuple(int(k) for k in LO))
         vd = gu.soa(v, tdt)
-        if op == VALUE:
-            pa.P2G.value(gu.ctx(), sd, vd, field)
-        elif op == GRADIENT:
-            pa.P2G.gradient(gu.ctx(), sd, vd, field)
-        else:
-            pa.P2G.divergence(gu.ctx(), sd, vd, field)
-        gu.ctx().synchronize()
+        ctx = gu.ctx()
+        ctx.synchronize()
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        try:
+            if op == VALUE:
+                pa.P2G.value(ctx, sd, vd, field)
+            elif op == GRADIENT:
+                pa.P2G.gradient(ctx, sd, vd, field)
+            else:
+                pa.P2G.divergence(ctx, sd, vd, field)
+            ctx.synchronize()
+            stale_runs = ctx.kernel_time("p2g_interp_stale")[1]
+        finally:
+            ctx.set_timing(False)
+        assert stale_runs == 1  # the brick path ran (its stale-list pass is timed)
         assert gu.rel_inf(gu.host(field.tensor), ref) < tol, (op, nc)
