@@ -304,6 +304,39 @@ int pic_permute(pic_context* ctx, int dtype, int64_t np, const uint32_t* perm,
                 int nfields, const pic_pfield* src, const pic_pfield* dst,
                 const int* ncomp);
 
+/* Particle initialisation, Picasso's initializeParticles
+ * (Picasso_ParticleInit.hpp:121-248 InitRandom, :279-400 InitUniform)
+ * without its creation functor.  Fills every cell of [cell_lo,
+ * cell_lo + ncell) (local cell indices; node i sits at mesh->low + i h):
+ *  - PIC_INIT_UNIFORM: ppc^3 particles per cell on the lattice
+ *    low + (0.5 + i_p) spacing, spacing = (high - low) / ppc per dimension
+ *    (:345-356); ppc is particles_per_cell_dim;
+ *  - PIC_INIT_RANDOM: ppc particles per cell, each coordinate drawn
+ *    uniformly in [low, high) (:193-197) from a counter-based generator of
+ *    (seed, particle, dimension) -- the same positions for the same seed,
+ *    whatever the device; the reference's Kokkos XorShift pool is only
+ *    specified by its distribution.
+ * Particle p = cell_id * n + local with cell_id = i + ncell[0] (j + ncell[1]
+ * k) (i fastest, :170-173) and, uniform, local = ip + ppc (jp + ppc kp)
+ * (:339-341).  x receives 3 components; volume (may be NULL) 1, the cell
+ * volume / n (:200).  Returns the number of particles written (<= capacity)
+ * or a negative status (PIC_ERR_CAPACITY when capacity is too small).  The
+ * creation functor's choice is then a keep mask: pic_select + pic_permute
+ * compact the fields (filterEmpties, :38-91). */
+enum { PIC_INIT_UNIFORM = 0, PIC_INIT_RANDOM = 1 };
+int64_t pic_init_particles(pic_context* ctx, int mode, int dtype, const pic_mesh* mesh,
+                           const int32_t cell_lo[3], const int32_t ncell[3], int ppc,
+                           uint64_t seed, int64_t capacity, const pic_pfield* x,
+                           const pic_pfield* volume);
+
+/* Stable selection (the compaction of Picasso's filterEmpties,
+ * Picasso_ParticleInit.hpp:38-91, which keeps the created particles but
+ * moves tail particles into the holes; here the order is kept): kept[0]
+ * receives the number of i < np with keep[i] != 0 and kept[1..] those i in
+ * increasing order.  kept holds np + 1 words.  pic_permute with
+ * perm = kept + 1 then gathers the particle fields. */
+int pic_select(pic_context* ctx, int64_t np, const uint8_t* keep, uint32_t* kept);
+
 /* Step graphs (no reference counterpart: Kokkos launches each kernel).
  * Between pic_graph_begin and pic_graph_end the context's calls are
  * captured into a HIP graph instead of running; pic_graph_launch replays

@@ -811,3 +811,54 @@ int orc_apic_step_omp(int order, const double low[3], double h,
             err |= 2;
     return err ? ORC_ERR_BOUNDS : ORC_OK;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Particle initialisation (Picasso_ParticleInit.hpp:121-248 random,
+ * :279-400 uniform), without the creation functor.  Particle
+ * p = cell_id * n + local, cell_id = i + ncell0 (j + ncell1 k) (:170-173);
+ * uniform local = ip + d (jp + d kp) (:339-341).  Node i sits at low + i h
+ * (LocalMesh::coordinates). */
+
+double orc_uniform01(uint64_t seed, uint64_t pid, int d)
+{
+    /* SplitMix64 (Steele, Lea, Flood 2014; Vigna's reference code) of the
+     * state seed + (3 pid + d + 1) * golden gamma; top 53 bits. */
+    uint64_t z = seed + (pid * 3u + (uint64_t)d + 1u) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+
+int64_t orc_init_particles(int mode, const double low[3], double h, const int32_t cell_lo[3],
+                           const int32_t ncell[3], int ppc, uint64_t seed, double* x,
+                           double* volume)
+{
+    if (!(h > 0.0) || ppc < 1 || (mode != 0 && mode != 1)) return ORC_ERR_ARG;
+    const int64_t npc = mode == 0 ? (int64_t)ppc * ppc * ppc : ppc;
+    const int64_t cells = (int64_t)ncell[0] * ncell[1] * ncell[2];
+    for (int64_t c = 0; c < cells; ++c) {
+        const int32_t own[3] = {(int32_t)(c % ncell[0]), (int32_t)((c / ncell[0]) % ncell[1]),
+                                (int32_t)(c / ((int64_t)ncell[0] * ncell[1]))};
+        double lc[3], hc[3];
+        for (int d = 0; d < 3; ++d) {
+            lc[d] = low[d] + (double)(cell_lo[d] + own[d]) * h;
+            hc[d] = low[d] + (double)(cell_lo[d] + own[d] + 1) * h;
+        }
+        for (int64_t l = 0; l < npc; ++l) {
+            const int64_t p = c * npc + l;
+            if (mode == 0) {
+                const int ip[3] = {(int)(l % ppc), (int)((l / ppc) % ppc), (int)(l / (ppc * ppc))};
+                for (int d = 0; d < 3; ++d) {
+                    const double sp = (hc[d] - lc[d]) / (double)ppc;
+                    x[3 * p + d] = 0.5 * sp + (double)ip[d] * sp + lc[d];
+                }
+            } else {
+                for (int d = 0; d < 3; ++d)
+                    x[3 * p + d] = lc[d] + (hc[d] - lc[d]) * orc_uniform01(seed, (uint64_t)p, d);
+            }
+            if (volume) volume[p] = h * h * h / (double)npc;
+        }
+    }
+    return cells * npc;
+}

@@ -118,6 +118,15 @@ int orc_g2p_gradu(int order, int entity, const double low[3], double h,
 int orc_grid_normalize(int64_t nn, int ncomp, const double* m,
                        const double* mu, double* u);
 
+/* Particle initialisation (Picasso_ParticleInit.hpp:121-248, 279-400)
+ * without the creation functor: mode 0 = uniform lattice of ppc^3 per cell,
+ * 1 = ppc random points per cell (coordinates orc_uniform01(seed, p, d));
+ * x[p][3], volume[p] (may be NULL).  Returns the particle count. */
+double orc_uniform01(uint64_t seed, uint64_t pid, int d);
+int64_t orc_init_particles(int mode, const double low[3], double h, const int32_t cell_lo[3],
+                           const int32_t ncell[3], int ppc, uint64_t seed, double* x,
+                           double* volume);
+
 /* Cell keys.  cell = floor((x - low) * (1/h) + dual_d/2) per dimension d
  * (dual_d = 0: mesh cells, 1: node-centred dual cells; dual = 0: none,
  * 1: every dimension, 0x10 | mask: dimension d when bit d is set, as

@@ -60,6 +60,11 @@ def _declare(L):
     L.orc_cell_keys.argtypes = [c_i64, _D, _D, c_double, _I32, _I32, _I32, c_int,
                                 _U32, _I32]
     L.orc_bin_sort.argtypes = [c_i64, _U32, ctypes.c_uint32, _U32, _U32]
+    L.orc_uniform01.argtypes = [ctypes.c_uint64, ctypes.c_uint64, c_int]
+    L.orc_uniform01.restype = c_double
+    L.orc_init_particles.argtypes = [c_int, _D, c_double, _I32, _I32, c_int, ctypes.c_uint64,
+                                     _D, _D]
+    L.orc_init_particles.restype = c_i64
     L.orc_det3.argtypes = [_D]
     L.orc_det3.restype = c_double
     L.orc_inverse3.argtypes = [_D, _D]
@@ -204,6 +209,23 @@ def cell_keys(x, low, h, cell_lo, ncell, brick=(1, 1, 1), dual=0):
                                _p(_i32(brick), _I32), int(dual), _p(keys, _U32),
                                _p(cells, _I32)), "cell_keys")
     return keys, cells
+
+
+def uniform01(seed, pid, d):
+    return lib().orc_uniform01(seed, pid, d)
+
+
+def init_particles(mode, low, h, cell_lo, ncell, ppc, seed=0):
+    """Positions [np, 3] and volumes [np] of initializeParticles without the
+    creation functor (mode 0 uniform lattice, 1 random)."""
+    npc = ppc ** 3 if mode == 0 else ppc
+    np_ = int(np.prod(ncell)) * npc
+    x = np.zeros((np_, 3))
+    vol = np.zeros(np_)
+    n = lib().orc_init_particles(mode, _p(_f64(low)), h, _p(_i32(cell_lo), _I32),
+                                 _p(_i32(ncell), _I32), ppc, seed, _p(x), _p(vol))
+    _check(n if n < 0 else 0, "init_particles")
+    return x, vol
 
 
 def bin_sort(keys, nkeys):

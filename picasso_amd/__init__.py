@@ -28,7 +28,8 @@ from ._abi import (CELL, EDGE_I, EDGE_J, EDGE_K, F32, F64, FACE_I, FACE_J, FACE_
 
 __all__ = ["Context", "Mesh", "GridField", "create_spline", "APIC", "PolyPIC", "FieldLocation",
            "grid_normalize", "g2p_gradu", "update_F", "bin_sort", "bin_count", "Binning",
-           "BinSort", "PicError", "DUAL_DIMS", "aligned_dual",
+           "BinSort", "PicError", "DUAL_DIMS", "aligned_dual", "InitUniform", "InitRandom",
+           "initialize_particles", "select", "filter_particles",
            "lib", "LIB_PATH"]
 
 LIB_PATH = _abi.LIB_PATH
@@ -483,22 +484,86 @@ class BinSort:
 
 
 def permute(ctx, perm, pairs):
-    """Gather particle fields into bin order: for each (src, dst) tensor pair
-    ([np, q] each, same dtype), dst[s] = src[perm[s]] (pic_permute)."""
+    """Gather particle fields: for each (src, dst) tensor pair ([n_src, q] and
+    [n, q], same dtype), dst[s] = src[perm[s]] for s < n (pic_permute); perm
+    holds n indices below n_src (a bin order, or a selection)."""
     if not pairs:
         return
     dt = _dtype_code(pairs[0][0])
-    np_ = pairs[0][0].shape[0]
+    np_ = pairs[0][1].shape[0]
     n = len(pairs)
     srcs = (_abi.PField * n)(*[_pfield(a, a.shape[1], "src") for a, _ in pairs])
     dsts = (_abi.PField * n)(*[_pfield(b, b.shape[1], "dst") for _, b in pairs])
     nc = (ctypes.c_int * n)(*[a.shape[1] for a, _ in pairs])
+    if perm.shape[0] < np_:
+        raise ValueError("permute: perm shorter than the destination")
     for a, b in pairs:
-        if _dtype_code(a) != dt or _dtype_code(b) != dt or a.shape != b.shape:
-            raise ValueError("permute: pairs must share dtype and shape")
+        if (_dtype_code(a) != dt or _dtype_code(b) != dt or a.shape[1] != b.shape[1]
+                or b.shape[0] != np_ or a.shape[0] < np_):
+            raise ValueError("permute: pairs must share dtype and width, destinations the "
+                             "same length, sources at least as long")
     rc = lib.pic_permute(ctx._bind_stream(), dt, np_, ctypes.c_void_p(perm.data_ptr()), n,
                          srcs, dsts, nc)
     check(rc, "pic_permute")
+
+
+InitUniform, InitRandom = 0, 1  # Picasso_ParticleInit.hpp:30-35 (tags)
+
+
+def initialize_particles(ctx, init, ppc, mesh, cell_lo, ncell, dtype=None, seed=0,
+                         with_volume=True):
+    """initializeParticles(InitUniform / InitRandom, ...)
+    (Picasso_ParticleInit.hpp:121-248, 279-400) without the creation
+    functor: positions x [np, 3] (and volumes [np, 1]) of every particle of
+    the cells [cell_lo, cell_lo + ncell), ppc per cell (InitRandom) or ppc^3
+    on a lattice (InitUniform; ppc = particles_per_cell_dim), in the
+    reference's particle order.  Apply the functor's choice with
+    filter_particles."""
+    import torch
+    dtype = dtype or torch.float64
+    n_cells = 1
+    for v in ncell:
+        n_cells *= int(v)
+    np_ = n_cells * (ppc ** 3 if init == InitUniform else ppc)
+    x = torch.empty((3, np_), dtype=dtype, device="cuda").t()
+    vol = torch.empty((1, np_), dtype=dtype, device="cuda").t() if with_volume else None
+    i3 = ctypes.c_int32 * 3
+    rc = lib.pic_init_particles(ctx._bind_stream(), int(init), _dtype_code(x),
+                                ctypes.byref(mesh.c()), i3(*[int(v) for v in cell_lo]),
+                                i3(*[int(v) for v in ncell]), int(ppc), int(seed) & (2**64 - 1),
+                                np_, ctypes.byref(_pfield(x, 3, "x")),
+                                ctypes.byref(_pfield(vol, 1, "volume")) if vol is not None
+                                else None)
+    check(rc, "pic_init_particles")
+    return (x, vol) if with_volume else x
+
+
+def select(ctx, keep):
+    """Indices of the particles a mask keeps, in increasing order, as a
+    uint32-valued int32 tensor (pic_select)."""
+    import torch
+    _require_device(keep, "keep")
+    k = keep.to(torch.uint8).contiguous()
+    np_ = k.shape[0]
+    kept = torch.empty(np_ + 1, dtype=torch.int32, device=k.device)
+    check(lib.pic_select(ctx._bind_stream(), np_, ctypes.c_void_p(k.data_ptr()),
+                         ctypes.c_void_p(kept.data_ptr())), "pic_select")
+    ctx.synchronize()
+    n = int(kept[0].item())
+    return kept[1:1 + n]
+
+
+def filter_particles(ctx, keep, *fields):
+    """Keep the particles a creation functor accepted (keep[i] != 0), in
+    order: filterEmpties (Picasso_ParticleInit.hpp:38-91) by pic_select +
+    pic_permute.  Returns new [n_kept, q] tensors, one per field."""
+    import torch
+    idx = select(ctx, keep)
+    outs = [torch.empty((f.shape[1], idx.shape[0]), dtype=f.dtype, device=f.device).t()
+            for f in fields]
+    if idx.shape[0] > 0:
+        permute(ctx, idx, list(zip(fields, outs)))
+    return outs
 
 
 def bin_count(geom):

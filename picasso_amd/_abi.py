@@ -93,6 +93,10 @@ PROTOTYPES = {
                                        _P(PField), _P(PField), _P(PField), _P(Bins)]),
     "pic_permute": (_c_int, [_vp, _c_int, _c_i64, _vp, _c_int, _P(PField), _P(PField),
                              _P(_c_int)]),
+    "pic_init_particles": (_c_i64, [_vp, _c_int, _c_int, _P(Mesh), _P(ctypes.c_int32),
+                                    _P(ctypes.c_int32), _c_int, ctypes.c_uint64, _c_i64,
+                                    _P(PField), _P(PField)]),
+    "pic_select": (_c_int, [_vp, _c_i64, _vp, _vp]),
     "pic_graph_begin": (_c_int, [_vp]),
     "pic_graph_end": (_c_int, [_vp, _P(_vp)]),
     "pic_graph_launch": (_c_int, [_vp, _vp]),

@@ -584,3 +584,43 @@ extern "C" int pic_permute(pic_context* ctx, int dtype, int64_t np, const uint32
     }
     return launch_status();
 }
+
+// ---------------------------------------------------------------------------
+// Stable selection: the list of the particles a mask keeps, in increasing
+// order, by the sort's tile-sum / scan-compact passes over 0/1 counts.
+namespace pic {
+
+__global__ void k_keep_counts(int64_t np, const uint8_t* keep, uint32_t* count)
+{
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i <= np;
+         i += int64_t(gridDim.x) * blockDim.x)
+        count[i] = i < np && keep[i] ? 1u : 0u;
+}
+
+}  // namespace pic
+
+extern "C" int pic_select(pic_context* ctx, int64_t np, const uint8_t* keep, uint32_t* kept)
+{
+    if (!ctx || np < 0 || np >= int64_t(UINT32_MAX) || !kept || (np > 0 && !keep))
+        return PIC_ERR_ARG;
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok) return PIC_ERR_HIP;
+    // scratch: count[np+1] | scan[np+1] | tile sums (64-bit)
+    const int64_t ntile = (np + 1 + SCAN_TILE - 1) / SCAN_TILE;
+    const int64_t head = (2 * (np + 1) + 1) & ~int64_t(1);  // tile sums 8-byte aligned
+    uint32_t* sc = static_cast<uint32_t*>(scratch(ctx, size_t(head + 2 * ntile) * 4));
+    if (!sc) return PIC_ERR_ALLOC;
+    uint32_t* count = sc;
+    uint32_t* scan = sc + np + 1;
+    uint64_t* tile_sums = reinterpret_cast<uint64_t*>(sc + head);
+    hipStream_t st = ctx->stream;
+    TimedLaunch tl(ctx, "select");
+    const unsigned b = (unsigned)std::min<int64_t>(blocks_for(np + 1, 256), 65536);
+    k_keep_counts<<<b, 256, 0, st>>>(np, keep, count);
+    k_tile_sums<<<(unsigned)ntile, SCAN_THREADS, 0, st>>>(count, np, tile_sums);
+    const bool prefixed = ntile > SCAN_TILE;
+    if (prefixed) k_scan_sums<<<1, SCAN_THREADS, 0, st>>>(tile_sums, ntile);
+    k_scan_compact<<<(unsigned)ntile, SCAN_THREADS, 0, st>>>(count, np, tile_sums, prefixed, scan,
+                                                             kept);
+    return launch_status();
+}

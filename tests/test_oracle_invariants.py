@@ -251,6 +251,56 @@ def test_aligned_dual_matches_the_knot_base():
             assert all(f in (0, 1) for f in flags)
 
 
+def test_uniform01_is_splitmix64():
+    """The random placement draws SplitMix64 outputs: for seed 0 the states
+    1, 2, 3 x golden gamma give the generator's published first outputs
+    0xe220a8397b1dcdaf, 0x6e789e6aa1b965f4, 0x06c45d188009454f."""
+    for d, v in enumerate((0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F)):
+        assert orpic.uniform01(0, 0, d) == (v >> 11) / 2.0 ** 53
+
+
+@pytest.mark.parametrize("ppd", [1, 2, 3])
+def test_init_uniform_lattice(ppd):
+    """InitUniform (Picasso_ParticleInit.hpp:279-400): each cell holds a
+    ppd^3 lattice with spacing h/ppd, half a spacing in from the cell faces,
+    centred on the cell; particle p belongs to cell p // ppd^3 with i
+    fastest and, inside, ip fastest; volumes add up to the cells' volume."""
+    cell_lo, ncell, h = (2, 1, 3), (3, 4, 2), 0.125
+    x, vol = orpic.init_particles(0, LOW, H, cell_lo, ncell, ppd)
+    npc = ppd ** 3
+    assert x.shape == (np.prod(ncell) * npc, 3)
+    c = np.arange(len(x)) // npc
+    own = np.stack([c % ncell[0], (c // ncell[0]) % ncell[1], c // (ncell[0] * ncell[1])], 1)
+    lc = LOW + (np.array(cell_lo) + own) * H
+    frac = (x - lc) / H
+    loc = np.arange(len(x)) % npc
+    ip = np.stack([loc % ppd, (loc // ppd) % ppd, loc // ppd ** 2], 1)
+    np.testing.assert_allclose(frac, (ip + 0.5) / ppd, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(x.reshape(-1, npc, 3).mean(1), lc[::npc] + 0.5 * H, atol=1e-12)
+    assert abs(vol.sum() - np.prod(ncell) * H ** 3) < 1e-12
+
+
+def test_init_random_in_cells():
+    """InitRandom (Picasso_ParticleInit.hpp:121-248): every particle lies in
+    its own cell (p // ppc, i fastest), the fractional coordinates are
+    uniform (mean 1/2, variance 1/12, no dimension correlated), the same
+    seed gives the same points and another seed different ones."""
+    cell_lo, ncell, ppc = (1, 2, 3), (8, 6, 5), 40
+    x, vol = orpic.init_particles(1, LOW, H, cell_lo, ncell, ppc, seed=7)
+    c = np.arange(len(x)) // ppc
+    own = np.stack([c % ncell[0], (c // ncell[0]) % ncell[1], c // (ncell[0] * ncell[1])], 1)
+    frac = (x - (LOW + (np.array(cell_lo) + own) * H)) / H
+    assert frac.min() >= 0.0 and frac.max() < 1.0
+    assert np.all(np.abs(frac.mean(0) - 0.5) < 0.01)
+    assert np.all(np.abs(frac.var(0) - 1 / 12) < 0.005)
+    assert np.all(np.abs(np.corrcoef(frac.T) - np.eye(3)) < 0.03)
+    x2, _ = orpic.init_particles(1, LOW, H, cell_lo, ncell, ppc, seed=7)
+    x3, _ = orpic.init_particles(1, LOW, H, cell_lo, ncell, ppc, seed=8)
+    np.testing.assert_array_equal(x, x2)
+    assert np.mean(x == x3) < 1e-3
+    np.testing.assert_allclose(vol, H ** 3 / ppc, rtol=1e-15)
+
+
 def test_cell_keys_out_of_range_is_error():
     x = np.array([[LOW[0] - 0.01, LOW[1], LOW[2]]])
     with pytest.raises(orpic.OracleError):
