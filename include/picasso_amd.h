@@ -227,6 +227,23 @@ int pic_polypic_g2p(pic_context* ctx, int order, int entity, int dtype,
                     const pic_pfield* x, const pic_grid* u_i,
                     const pic_pfield* c_p, const pic_bins* bins);
 
+/* Grid halo on one GPU: the gather and scatter-sum halo steps GridOperator
+ * runs around each apply (Picasso_GridOperator.hpp:203-215, 465-491;
+ * Cabana::Grid::Halo, not in the reference tree), for periodic dimensions.
+ * The window of g holds the owned nodes [own_lo, own_lo + own_n) and ghost
+ * layers; in a dimension with periodic[d] != 0 node i is the image of
+ * own_lo + ((i - own_lo) mod own_n), the ghost layers together no wider
+ * than own_n.
+ *  - PIC_HALO_SCATTER_SUM: every ghost adds its ncomp values into its
+ *    owned image (after a p2g, the ghosts' partial sums).  Ghost values
+ *    are unspecified afterwards (a gather restores them).
+ *  - PIC_HALO_GATHER: every ghost takes its owned image's values (before a
+ *    g2p).
+ * Non-periodic dimensions exchange nothing (no neighbour on one GPU). */
+enum { PIC_HALO_GATHER = 0, PIC_HALO_SCATTER_SUM = 1 };
+int pic_grid_halo(pic_context* ctx, int op, int dtype, const pic_grid* g, int ncomp,
+                  const int32_t own_lo[3], const int32_t own_n[3], const int32_t periodic[3]);
+
 /* Grid velocity u_i = mu_i / m_i over the window of m_i (no reference
  * function; doc/basic/explict_nearly_incompressible_flow.md:18-23 without
  * forces).  Nodes with m_i == 0 get u_i = 0.  For PIC_F32, nodes with

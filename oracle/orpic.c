@@ -862,3 +862,56 @@ int64_t orc_init_particles(int mode, const double low[3], double h, const int32_
     }
     return cells * npc;
 }
+
+/* ---------------------------------------------------------------------- */
+/* Grid halo on one rank, periodic dimensions (the gather / scatter-sum
+ * steps of Picasso_GridOperator.hpp:203-215, 465-491).  Window [lo, lo+n),
+ * owned [olo, olo+on); node i of a periodic dimension is the image of
+ * olo + ((i - olo) mod on).  op 1: scatter-sum, folding one periodic
+ * dimension after the other; op 0: gather.  g[(i*n1 + j)*n2 + k][ncomp]. */
+
+static int32_t orc_wrap(int32_t i, int32_t lo, int32_t n)
+{
+    int32_t r = (i - lo) % n;
+    return lo + (r < 0 ? r + n : r);
+}
+
+int orc_grid_halo(int op, const int32_t lo[3], const int32_t n[3], int ncomp, double* g,
+                  const int32_t olo[3], const int32_t on[3], const int32_t periodic[3])
+{
+    for (int d = 0; d < 3; ++d)
+        if (on[d] < 1 || olo[d] < lo[d] || olo[d] + on[d] > lo[d] + n[d]) return ORC_ERR_ARG;
+#define AT(i, j, k) (((int64_t)((i) - lo[0]) * n[1] + ((j) - lo[1])) * n[2] + ((k) - lo[2]))
+    if (op == 1) {
+        for (int d = 0; d < 3; ++d) {
+            if (!periodic[d]) continue;
+            for (int32_t i = lo[0]; i < lo[0] + n[0]; ++i)
+                for (int32_t j = lo[1]; j < lo[1] + n[1]; ++j)
+                    for (int32_t k = lo[2]; k < lo[2] + n[2]; ++k) {
+                        int32_t idx[3] = {i, j, k};
+                        if (idx[d] >= olo[d] && idx[d] < olo[d] + on[d]) continue;
+                        const int64_t src = AT(i, j, k);
+                        idx[d] = orc_wrap(idx[d], olo[d], on[d]);
+                        const int64_t dst = AT(idx[0], idx[1], idx[2]);
+                        for (int c = 0; c < ncomp; ++c) g[dst * ncomp + c] += g[src * ncomp + c];
+                    }
+        }
+    } else {
+        for (int32_t i = lo[0]; i < lo[0] + n[0]; ++i)
+            for (int32_t j = lo[1]; j < lo[1] + n[1]; ++j)
+                for (int32_t k = lo[2]; k < lo[2] + n[2]; ++k) {
+                    const int32_t idx[3] = {i, j, k};
+                    int32_t img[3];
+                    int ghost = 0;
+                    for (int d = 0; d < 3; ++d) {
+                        img[d] = periodic[d] ? orc_wrap(idx[d], olo[d], on[d]) : idx[d];
+                        ghost |= img[d] != idx[d];
+                    }
+                    if (!ghost) continue;
+                    const int64_t src = AT(img[0], img[1], img[2]), dst = AT(i, j, k);
+                    for (int c = 0; c < ncomp; ++c) g[dst * ncomp + c] = g[src * ncomp + c];
+                }
+    }
+#undef AT
+    return ORC_OK;
+}

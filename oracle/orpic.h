@@ -127,6 +127,15 @@ int64_t orc_init_particles(int mode, const double low[3], double h, const int32_
                            const int32_t ncell[3], int ppc, uint64_t seed, double* x,
                            double* volume);
 
+/* Grid halo on one rank for periodic dimensions (GridOperator's gather /
+ * scatter-sum, Picasso_GridOperator.hpp:203-215, 465-491): op 1 adds every
+ * ghost into its image olo + ((i - olo) mod on), one periodic dimension
+ * after the other (ghost values are unspecified afterwards); op 0 copies
+ * images into ghosts.  g is [n0][n1][n2][ncomp]
+ * over the window [lo, lo+n). */
+int orc_grid_halo(int op, const int32_t lo[3], const int32_t n[3], int ncomp, double* g,
+                  const int32_t olo[3], const int32_t on[3], const int32_t periodic[3]);
+
 /* Cell keys.  cell = floor((x - low) * (1/h) + dual_d/2) per dimension d
  * (dual_d = 0: mesh cells, 1: node-centred dual cells; dual = 0: none,
  * 1: every dimension, 0x10 | mask: dimension d when bit d is set, as

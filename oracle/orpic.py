@@ -60,6 +60,7 @@ def _declare(L):
     L.orc_cell_keys.argtypes = [c_i64, _D, _D, c_double, _I32, _I32, _I32, c_int,
                                 _U32, _I32]
     L.orc_bin_sort.argtypes = [c_i64, _U32, ctypes.c_uint32, _U32, _U32]
+    L.orc_grid_halo.argtypes = [c_int, _I32, _I32, c_int, _D, _I32, _I32, _I32]
     L.orc_uniform01.argtypes = [ctypes.c_uint64, ctypes.c_uint64, c_int]
     L.orc_uniform01.restype = c_double
     L.orc_init_particles.argtypes = [c_int, _D, c_double, _I32, _I32, c_int, ctypes.c_uint64,
@@ -209,6 +210,16 @@ def cell_keys(x, low, h, cell_lo, ncell, brick=(1, 1, 1), dual=0):
                                _p(_i32(brick), _I32), int(dual), _p(keys, _U32),
                                _p(cells, _I32)), "cell_keys")
     return keys, cells
+
+
+def grid_halo(op, lo, g, own_lo, own_n, periodic):
+    """op 1 scatter-sum, 0 gather, in place on g [n0, n1, n2(, ncomp)]."""
+    assert g.flags.c_contiguous and g.dtype == np.float64
+    ncomp = 1 if g.ndim == 3 else g.shape[3]
+    _check(lib().orc_grid_halo(op, _p(_i32(lo), _I32), _p(_i32(g.shape[:3]), _I32), ncomp,
+                               _p(g), _p(_i32(own_lo), _I32), _p(_i32(own_n), _I32),
+                               _p(_i32([int(bool(v)) for v in periodic]), _I32)), "grid_halo")
+    return g
 
 
 def uniform01(seed, pid, d):

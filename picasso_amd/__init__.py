@@ -29,7 +29,8 @@ from ._abi import (CELL, EDGE_I, EDGE_J, EDGE_K, F32, F64, FACE_I, FACE_J, FACE_
 __all__ = ["Context", "Mesh", "GridField", "create_spline", "APIC", "PolyPIC", "FieldLocation",
            "grid_normalize", "g2p_gradu", "update_F", "bin_sort", "bin_count", "Binning",
            "BinSort", "PicError", "DUAL_DIMS", "aligned_dual", "InitUniform", "InitRandom",
-           "initialize_particles", "select", "filter_particles",
+           "initialize_particles", "select", "filter_particles", "halo_scatter",
+           "halo_gather",
            "lib", "LIB_PATH"]
 
 LIB_PATH = _abi.LIB_PATH
@@ -352,6 +353,30 @@ def grid_normalize(ctx, m_i, mu_i, u_i, rel_mass_eps=0.0):
                                 ctypes.byref(m_i.c()), ctypes.byref(mu_i.c()),
                                 ctypes.byref(u_i.c()), float(rel_mass_eps))
     check(rc, "pic_grid_normalize")
+
+
+HALO_GATHER, HALO_SCATTER_SUM = 0, 1
+
+
+def _halo(ctx, op, field, own_lo, own_n, periodic):
+    i3 = ctypes.c_int32 * 3
+    rc = lib.pic_grid_halo(ctx._bind_stream(), op, _dtype_code(field.tensor), ctypes.byref(field.c()),
+                           field.ncomp, i3(*[int(v) for v in own_lo]),
+                           i3(*[int(v) for v in own_n]), i3(*[int(bool(v)) for v in periodic]))
+    check(rc, "pic_grid_halo")
+
+
+def halo_scatter(ctx, field, own_lo, own_n, periodic=(True, True, True)):
+    """Scatter-sum halo (GridOperator after a p2g, Picasso_GridOperator.hpp:
+    211-215, 489-491) on one GPU: every ghost node of a periodic dimension
+    adds into its owned image own_lo + (i - own_lo) mod own_n."""
+    _halo(ctx, HALO_SCATTER_SUM, field, own_lo, own_n, periodic)
+
+
+def halo_gather(ctx, field, own_lo, own_n, periodic=(True, True, True)):
+    """Gather halo (GridOperator before an apply, Picasso_GridOperator.hpp:
+    203-208, 465-466) on one GPU: every ghost node takes its owned image."""
+    _halo(ctx, HALO_GATHER, field, own_lo, own_n, periodic)
 
 
 def g2p_gradu(ctx, u_i, gradu, sd):

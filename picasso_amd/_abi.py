@@ -97,6 +97,8 @@ PROTOTYPES = {
                                     _P(ctypes.c_int32), _c_int, ctypes.c_uint64, _c_i64,
                                     _P(PField), _P(PField)]),
     "pic_select": (_c_int, [_vp, _c_i64, _vp, _vp]),
+    "pic_grid_halo": (_c_int, [_vp, _c_int, _c_int, _P(Grid), _c_int, _P(ctypes.c_int32),
+                               _P(ctypes.c_int32), _P(ctypes.c_int32)]),
     "pic_graph_begin": (_c_int, [_vp]),
     "pic_graph_end": (_c_int, [_vp, _P(_vp)]),
     "pic_graph_launch": (_c_int, [_vp, _vp]),

@@ -251,6 +251,37 @@ def test_aligned_dual_matches_the_knot_base():
             assert all(f in (0, 1) for f in flags)
 
 
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (1, 0, 1), (0, 1, 0), (0, 0, 0)])
+def test_grid_halo_scatter_is_the_sum_over_preimages(periodic):
+    """The sequential per-dimension fold equals the direct definition: each
+    owned-in-the-periodic-dimensions node receives the sum of every window
+    node whose image (all periodic dimensions mapped at once) it is; the
+    total over those nodes is conserved; gather copies the images.  (Ghost
+    values after the scatter are unspecified: an x-fold adds into nodes that
+    are still y-ghosts.)"""
+    rng = np.random.default_rng(sum(periodic))
+    lo, n = np.array([-2, -1, -3]), np.array([14, 11, 15])
+    olo, on = np.array([0, 1, 0]), np.array([9, 8, 10])
+    g = rng.normal(size=tuple(n) + (2,))
+    out = orpic.grid_halo(1, lo, g.copy(), olo, on, periodic)
+    idx = np.stack(np.meshgrid(*[np.arange(lo[d], lo[d] + n[d]) for d in range(3)],
+                               indexing="ij"), -1).reshape(-1, 3)
+    img = idx.copy()
+    for d in range(3):
+        if periodic[d]:
+            img[:, d] = olo[d] + np.mod(idx[:, d] - olo[d], on[d])
+    ghost = np.any(img != idx, 1)
+    want = g.reshape(-1, 2).copy()
+    flat = lambda v: ((v[:, 0] - lo[0]) * n[1] + (v[:, 1] - lo[1])) * n[2] + (v[:, 2] - lo[2])  # noqa: E731
+    np.add.at(want, flat(img[ghost]), g.reshape(-1, 2)[ghost])
+    keep = ~ghost  # nodes that are their own image: owned in every periodic dimension
+    np.testing.assert_allclose(out.reshape(-1, 2)[keep], want[keep], rtol=1e-13, atol=1e-13)
+    np.testing.assert_allclose(out.reshape(-1, 2)[keep].sum(0), g.reshape(-1, 2).sum(0),
+                               rtol=1e-12)
+    ga = orpic.grid_halo(0, lo, out.copy(), olo, on, periodic)
+    np.testing.assert_array_equal(ga.reshape(-1, 2), out.reshape(-1, 2)[flat(img)])
+
+
 def test_uniform01_is_splitmix64():
     """The random placement draws SplitMix64 outputs: for seed 0 the states
     1, 2, 3 x golden gamma give the generator's published first outputs
