@@ -7,6 +7,10 @@
 //   Picasso::PolyPIC::g2p(u_i, c_p, sd)                  (Picasso_PolyPIC.hpp:235-243)
 //   Picasso::G2P::value / gradient / divergence, Picasso::P2G::value /
 //   gradient / divergence   (Picasso_ParticleInterpolation.hpp:110-279)
+//   Picasso::initializeParticles(InitUniform / InitRandom, ...)
+//                           (Picasso_ParticleInit.hpp:121-248, 279-400)
+//   GridOperator's gather / scatter-sum halos on one GPU, periodic
+//                           (Picasso_GridOperator.hpp:203-215, 465-491)
 //
 // The reference calls these once per particle inside a Kokkos kernel; here
 // each call covers every particle of the spline data at once (the batch form
@@ -212,6 +216,59 @@ inline void g2p(pic_context* ctx, const pic_grid& u_i, const pic_pfield& c_p,
 }
 
 }  // namespace PolyPIC
+
+// Particle initialisation without the creation functor: positions (and
+// volumes) of every particle of the cells [cell_lo, cell_lo + ncell); the
+// functor's choice then goes through select + pic_permute.  Returns the
+// number of particles written.
+struct InitUniform {};
+struct InitRandom {};
+
+inline int64_t initializeParticles(pic_context* ctx, InitUniform, int particles_per_cell_dim,
+                                   int dtype, const pic_mesh& mesh, const int32_t cell_lo[3],
+                                   const int32_t ncell[3], int64_t capacity,
+                                   const pic_pfield& x, const pic_pfield* volume = nullptr)
+{
+    const int64_t n = pic_init_particles(ctx, PIC_INIT_UNIFORM, dtype, &mesh, cell_lo, ncell,
+                                         particles_per_cell_dim, 0, capacity, &x, volume);
+    if (n < 0) check(int(n));
+    return n;
+}
+
+inline int64_t initializeParticles(pic_context* ctx, InitRandom, int particles_per_cell,
+                                   int dtype, const pic_mesh& mesh, const int32_t cell_lo[3],
+                                   const int32_t ncell[3], int64_t capacity,
+                                   const pic_pfield& x, const pic_pfield* volume = nullptr,
+                                   uint64_t seed = 0)
+{
+    const int64_t n = pic_init_particles(ctx, PIC_INIT_RANDOM, dtype, &mesh, cell_lo, ncell,
+                                         particles_per_cell, seed, capacity, &x, volume);
+    if (n < 0) check(int(n));
+    return n;
+}
+
+// kept[0] = number of kept particles, kept[1..] their indices in order.
+inline void select(pic_context* ctx, int64_t np, const uint8_t* keep, uint32_t* kept)
+{
+    check(pic_select(ctx, np, keep, kept));
+}
+
+namespace Halo {
+
+inline void scatterSum(pic_context* ctx, int dtype, const pic_grid& g, int ncomp,
+                       const int32_t own_lo[3], const int32_t own_n[3],
+                       const int32_t periodic[3])
+{
+    check(pic_grid_halo(ctx, PIC_HALO_SCATTER_SUM, dtype, &g, ncomp, own_lo, own_n, periodic));
+}
+
+inline void gather(pic_context* ctx, int dtype, const pic_grid& g, int ncomp,
+                   const int32_t own_lo[3], const int32_t own_n[3], const int32_t periodic[3])
+{
+    check(pic_grid_halo(ctx, PIC_HALO_GATHER, dtype, &g, ncomp, own_lo, own_n, periodic));
+}
+
+}  // namespace Halo
 
 }  // namespace picasso_amd
 
