@@ -88,8 +88,10 @@ __device__ __forceinline__ uint32_t wave_agg_resolve(const AggTicket& t)
     return uint32_t(__shfl((int)t.base, t.leader)) + t.rank;
 }
 
-// Particles per work-item in the per-particle sort passes: their loads are
-// all in flight together, and the grid is a quarter of the particles.
+// Particles per work-item in the per-particle sort passes (the grid is a
+// quarter of the particles).  The key pass waits for each item's position
+// before loading the next; at 40 registers, eight waves per SIMD hide that
+// (issuing all twelve loads first measured no faster).
 constexpr int SORT_ITEMS = 4;
 constexpr int SORT_PT = 256;  // work-items per block of those passes
 
