@@ -122,6 +122,49 @@ def test_p2g_interp_zero_first_and_accumulate(pa):
     assert gu.rel_inf(gu.host(field.tensor), ref) < 1e-12
 
 
+@pytest.mark.parametrize("binned", [False, True])
+def test_grid_operator_reset_then_accumulate(pa, binned):
+    """GridOperator's scatter semantics as tstGridOperator3d.hpp:390-429 pins
+    them: scatter fields holding wrong data (-1.1, -2.2) are reset to zero
+    and then hold ppc * value per cell; gather fields (2.0, 3.0) reach every
+    particle unchanged.  Particles sit at cell centres (ppc copies each) on
+    the Cell lattice with linear splines, so every weight is exactly 1."""
+    import gpu_util as gu
+    ppc = 3
+    cells = np.stack(np.meshgrid(*[np.arange(4, 12)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    x = np.repeat(LOW + H * (cells + 0.5), ppc, axis=0)
+    P = len(x)
+    mesh = pa.Mesh(LOW, H)
+    xd = gu.soa(x)
+    bins = None
+    if binned:
+        bins = pa.bin_sort(gu.ctx(), mesh, xd,
+                           pa.Binning((3, 3, 3), (12, 12, 12), (4, 4, 4),
+                                      pa.aligned_dual(1, pa.FieldLocation.Cell)))
+    sd = pa.create_spline(pa.FieldLocation.Cell, 1, mesh, xd, bins)
+    shp = tuple(int(k) for k in N)
+    lo = tuple(int(k) for k in LO)
+    foo_out = pa.GridField(gu.dev(np.full(shp + (3,), -1.1)), lo)
+    bar_out = pa.GridField(gu.dev(np.full(shp + (1,), -2.2)), lo)
+    pa.P2G.value(gu.ctx(), sd, gu.soa(np.full((P, 3), 2.0)), foo_out, zero_first=True)
+    pa.P2G.value(gu.ctx(), sd, gu.soa(np.full((P, 1), 3.0)), bar_out, zero_first=True)
+    foo_p = gu.dev(np.zeros((P, 3)))
+    bar_p = gu.dev(np.zeros((P, 1)))
+    pa.G2P.value(gu.ctx(), sd, pa.GridField(gu.dev(np.full(shp + (3,), 2.0)), lo), foo_p)
+    pa.G2P.value(gu.ctx(), sd, pa.GridField(gu.dev(np.full(shp + (1,), 3.0)), lo), bar_p)
+    gu.ctx().synchronize()
+    np.testing.assert_array_equal(gu.host(foo_p), 2.0)
+    np.testing.assert_array_equal(gu.host(bar_p), 3.0)
+    own = tuple(slice(4 - LO[d], 12 - LO[d]) for d in range(3))
+    foo = gu.host(foo_out.tensor)
+    bar = gu.host(bar_out.tensor)
+    np.testing.assert_array_equal(foo[own], ppc * 2.0)
+    np.testing.assert_array_equal(bar[own], ppc * 3.0)
+    mask = np.ones(shp, bool)
+    mask[own] = False
+    assert np.all(foo[mask] == 0.0) and np.all(bar[mask] == 0.0)
+
+
 def test_interp_rejects_unsupported_shapes(pa):
     import gpu_util as gu
     x = LOW + H * np.full((4, 3), 6.5)
