@@ -165,6 +165,68 @@ def test_grid_operator_reset_then_accumulate(pa, binned):
     assert np.all(foo[mask] == 0.0) and np.all(bar[mask] == 0.0)
 
 
+@pytest.mark.parametrize("binned", [False, True])
+def test_reference_interpolation_test_periodic_box(pa, binned):
+    """tstParticleInterpolation.hpp:351-600, on this library's pieces: a
+    periodic 18 x 22 x 39 box of 0.05 cells with one ghost layer
+    (particle_interpolation_test.json), one particle at every cell centre
+    (InitUniform, ppc 1), linear node splines.  Constant particle values
+    3.5 give 3.5 at every owned node after P2G and the scatter-sum halo, and
+    zero gradients and divergences; a constant grid 3.5 (halo gathered)
+    gives 3.5 at every particle and zero gradients and divergences."""
+    import gpu_util as gu
+    h = 0.05
+    nc = np.array([18, 22, 39])
+    low = np.array([-1.2, 0.1, 1.1]) - h  # one ghost cell
+    mesh = pa.Mesh(low, h)
+    ctx = gu.ctx()
+    x, _ = pa.initialize_particles(ctx, pa.InitUniform, 1, mesh, (1, 1, 1), nc,
+                                   dtype=torch.float64)
+    P = x.shape[0]
+    assert P == int(nc.prod())
+    bins = None
+    if binned:
+        bins = pa.bin_sort(ctx, mesh, x, pa.Binning((1, 1, 1), tuple(int(v) for v in nc),
+                                                    (4, 4, 4), 0))
+    sd = pa.create_spline(pa.FieldLocation.Node, 1, mesh, x, bins)
+    shp = tuple(int(v) + 2 for v in nc)  # nodes 0 .. nc + 1: owned 1 .. nc, one ghost each side
+    own_lo, own_n = (1, 1, 1), tuple(int(v) for v in nc)
+    own = (slice(1, 1 + nc[0]), slice(1, 1 + nc[1]), slice(1, 1 + nc[2]))
+
+    def node_field(ncomp, value=0.0):
+        return pa.GridField(gu.dev(np.full(shp + (ncomp,), value)))
+
+    def p2g(op, pv, ncomp_field):
+        f = node_field(ncomp_field)
+        getattr(pa.P2G, op)(ctx, sd, gu.dev(pv), f, zero_first=True)
+        pa.halo_scatter(ctx, f, own_lo, own_n)
+        ctx.synchronize()
+        return gu.host(f.tensor)[own]
+
+    np.testing.assert_allclose(p2g("value", np.full((P, 1), 3.5), 1), 3.5, rtol=1e-12)
+    np.testing.assert_allclose(p2g("value", np.full((P, 3), 3.5), 3), 3.5, rtol=1e-12)
+    np.testing.assert_allclose(p2g("gradient", np.full((P, 1), 3.5), 3), 0.0, atol=1e-12)
+    np.testing.assert_allclose(p2g("divergence", np.full((P, 3), 3.5), 1), 0.0, atol=1e-12)
+    np.testing.assert_allclose(p2g("divergence", np.full((P, 9), 3.5), 3), 0.0, atol=1e-12)
+
+    def g2p(op, ncomp_field, nout, ncomp):
+        f = node_field(ncomp_field, 3.5)
+        pa.halo_gather(ctx, f, own_lo, own_n)
+        out = gu.dev(np.zeros((P, nout)))
+        if op == "divergence":
+            pa.G2P.divergence(ctx, sd, f, out)
+        else:
+            getattr(pa.G2P, op)(ctx, sd, f, out, ncomp=ncomp)
+        ctx.synchronize()
+        return gu.host(out)
+
+    np.testing.assert_allclose(g2p("value", 1, 1, 1), 3.5, rtol=1e-12)
+    np.testing.assert_allclose(g2p("value", 3, 3, 3), 3.5, rtol=1e-12)
+    np.testing.assert_allclose(g2p("gradient", 1, 3, 1), 0.0, atol=1e-12)
+    np.testing.assert_allclose(g2p("gradient", 3, 9, 3), 0.0, atol=1e-12)
+    np.testing.assert_allclose(g2p("divergence", 3, 1, 3), 0.0, atol=1e-12)
+
+
 def test_interp_rejects_unsupported_shapes(pa):
     import gpu_util as gu
     x = LOW + H * np.full((4, 3), 6.5)
