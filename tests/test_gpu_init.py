@@ -89,3 +89,40 @@ def test_initialize_and_filter_like_a_creation_functor(pa, dtype):
     assert 0 < rkeep.sum() < len(rkeep)
     np.testing.assert_array_equal(gu.host(xk), cast(rx)[rkeep])
     np.testing.assert_array_equal(gu.host(vk)[:, 0], cast(rvol)[rkeep])
+
+
+@pytest.mark.parametrize("init, ppc, boundary, multiplier",
+                         [(1, 17, 1, 1), (1, 9, 3, 1), (0, 3, 1, 1), (0, 2, 2, 1),
+                          (0, 3, 1, 3), (0, 2, 3, 3), (1, 11, 1, 4), (1, 4, 4, 3)])
+def test_reference_particle_init_test(pa, init, ppc, boundary, multiplier):
+    """tstParticleInit.hpp:46-200 on this library: a 14 x 17 x 19 mesh of 0.23
+    cells at (1.2, 3.3, -2.8) with two ghost layers
+    (particle_init_test.json); the creation functor keeps particles strictly
+    inside the box `boundary` cells in from every face; initialising
+    `multiplier` times (a new seed each time) appends.  Expected: exactly
+    multiplier * n * (cells left in the box) particles, all strictly inside
+    the box, each with volume h^3 / n (n = ppc^3 uniform, ppc random)."""
+    import gpu_util as gu
+    h = 0.23
+    nc = np.array([14, 17, 19])
+    glow = np.array([1.2, 3.3, -2.8])
+    ghigh = glow + h * nc
+    mesh = pa.Mesh(glow - 2 * h, h)
+    lo_box, hi_box = glow + h * boundary, ghigh - h * boundary
+    xs, vs = [], []
+    for it in range(multiplier):
+        x, vol = pa.initialize_particles(gu.ctx(), init, ppc, mesh, (2, 2, 2), nc,
+                                         dtype=torch.float64, seed=17 + it)
+        lo_t = torch.as_tensor(lo_box, device=gu.DEV)
+        hi_t = torch.as_tensor(hi_box, device=gu.DEV)
+        keep = ((x > lo_t) & (x < hi_t)).all(1)
+        xk, vk = pa.filter_particles(gu.ctx(), keep, x, vol)
+        xs.append(gu.host(xk))
+        vs.append(gu.host(vk)[:, 0])
+    x = np.concatenate(xs)
+    v = np.concatenate(vs)
+    n = ppc ** 3 if init == 0 else ppc
+    assert len(x) > 0
+    assert len(x) == multiplier * n * int(np.prod(nc - 2 * boundary))
+    assert np.all(x > lo_box) and np.all(x < hi_box)
+    np.testing.assert_allclose(v, h ** 3 / n, rtol=1e-15)
