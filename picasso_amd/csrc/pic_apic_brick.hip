@@ -88,6 +88,10 @@ struct P2GShape {
     // f0..f2 (rows 0-2, written from registers), m (row ROW_M) and the 12
     // scalars of c_p in storage order (rows ROW_C..ROW_C+11).
     static constexpr bool DMA = sizeof(T) == 4;
+    // FP32 quadratic: the staged v_d rows are replaced, once per particle,
+    // by U_d = v_d - h D^-1 sum_a B_da f_a, which leaves the accumulation
+    // loop fewer vector instructions per particle (p2g_component_pre).
+    static constexpr bool PRE = DMA && ORDER == 2;
     static constexpr int ROW_M = 3, ROW_C = 4;
     static constexpr int STAGE = CAP * 16;                   // scalars
     static constexpr int SUMS = CPR * SLOTS;
@@ -153,6 +157,31 @@ __device__ __forceinline__ void p2g_component(const Spline<ORDER, T>& sd, T m, T
     }
 }
 
+// p2g_component of a quadratic spline with wm = m w_i(f0), w1 = w_j(f1),
+// w2 = w_k(f2), and U = v - h D^-1 (B0 f0 + B1 f1 + B2 f2) formed at staging.
+// With d_a = h (o_a - f_a), o = (i, j, k) - 1, the APIC sum
+// m w_i w_j w_k (v + D^-1 (B0 d0 + B1 d1 + B2 d2)) (Picasso_APIC.hpp:110-138)
+// is m w_i w_j w_k (U + h D^-1 (B0 o_0 + B1 o_1 + B2 o_2)).
+__device__ __forceinline__ void p2g_component_pre(const float wm[3], const float w1[3],
+                                                  const float w2[3], float U, float B0, float B1,
+                                                  float B2, float hDinv, float* acc)
+{
+    const float e0 = hDinv * B0, e1 = hDinv * B1, e2 = hDinv * B2;
+    const float ai[3] = {U - e0, U, U + e0};
+    const float wck0 = -(w2[0] * e2), wck2 = w2[2] * e2;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float P = wm[i] * w1[j];
+            const float Tij = j == 0 ? ai[i] - e1 : (j == 1 ? ai[i] : ai[i] + e1);
+            float* a = acc + (i * 3 + j) * 3;
+            a[0] += P * (w2[0] * Tij + wck0);
+            a[1] += P * (w2[1] * Tij);
+            a[2] += P * (w2[2] * Tij + wck2);
+        }
+}
+
 template <int ORDER, typename T>
 using P2GState = BrickStage<(P2GShape<ORDER, T>::CAP + P2GShape<ORDER, T>::NT - 1) /
                                 P2GShape<ORDER, T>::NT,
@@ -182,6 +211,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
     const int grp = STAG ? 1 + a.sdim : (comp > 0 ? comp : 1);
     const T hh = (T)a.lat.h;
     const T Dinv = T(ORDER == 3 ? 3.0 : 4.0) / (hh * hh);  // APIC::inertialScaling (:33-47)
+    const T hDinv = hh * Dinv;
     T acc[NN];
 #pragma unroll
     for (int n = 0; n < NN; ++n) acc[n] = T(0);
@@ -215,6 +245,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
 #pragma unroll
                 for (int e = 0; e < 12; ++e) load_to_lds(&a.c.at(cb, e), row + (S::ROW_C + e) * CAP);
             }
+            T frk[K][3];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
                 const uint32_t s = c0 + threadIdx.x + uint32_t(k * NT);
@@ -222,7 +253,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
                 const int q = int(s - c0);
                 const int lb = int(st.cs.key[k] - kb);
                 const int eb[3] = {f0[0] + lb / (PB * PB), f0[1] + (lb / PB) % PB, f0[2] + lb % PB};
-                T fr[3];
+                T* fr = frk[k];
                 bool ok = true;
 #pragma unroll
                 for (int dim = 0; dim < 3; ++dim) {
@@ -241,6 +272,22 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
                 st.cs.template note<BINS>(k, q, kb, rng);
             }
             __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
+            if constexpr (S::PRE) {
+                // U_d over v_d; this work-item's own copies have landed.
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    const uint32_t s = c0 + threadIdx.x + uint32_t(k * NT);
+                    if (s >= c1) continue;
+                    T* cr = lds + S::ROW_C * CAP + int(s - c0);
+                    const T* fr = frk[k];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const T sB = cr[(3 + d) * CAP] * fr[0] + cr[(6 + d) * CAP] * fr[1] +
+                                     cr[(9 + d) * CAP] * fr[2];
+                        cr[d * CAP] = cr[d * CAP] - hDinv * sB;
+                    }
+                }
+            }
         } else {
         // Stage the chunk: consecutive work-items, consecutive particles.
 #pragma unroll
@@ -286,6 +333,31 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
         if (c0 == beg) w.fetch();
         const uint32_t q0 = rng[l], q1 = rng[BINS + l];
         for (uint32_t q = q0; q < q1; ++q) {
+            if constexpr (S::PRE) {
+                const T x0 = lds[q];
+                if (x0 == T(STALE_F)) continue;
+                T wm[3], w1[3], w2[3];
+                bspline_w<2, T>(x0, wm);
+                bspline_w<2, T>(lds[CAP + q], w1);
+                bspline_w<2, T>(lds[2 * CAP + q], w2);
+                const T mm = lds[S::ROW_M * CAP + q];
+#pragma unroll
+                for (int i = 0; i < 3; ++i) wm[i] *= mm;
+                // U_d = row d, B(d, a) = c(1 + a, d) = row 3 + 3a + d of
+                // the c block; the mass work-item uses U = 1, B = 0.
+                const T* cr = lds + (S::ROW_C + grp - 1) * CAP + q;
+                T4 ub;
+                ub.x = cr[0];
+                ub.y = cr[3 * CAP];
+                ub.z = cr[6 * CAP];
+                ub.w = cr[9 * CAP];
+                if (comp == 0) {
+                    ub.x = T(1);
+                    ub.y = ub.z = ub.w = T(0);
+                }
+                p2g_component_pre(wm, w1, w2, ub.x, ub.y, ub.z, ub.w, hDinv, acc);
+                continue;
+            }
             T4 fm, ub;
             if constexpr (S::DMA) {
                 fm.x = lds[q];
