@@ -148,7 +148,7 @@ __device__ __forceinline__ void p2g_interp_particle(const InterpArgs<T>& a, int6
 // ---------------------------------------------------------------------------
 // P2G through 4^3-bin bricks, for particles binned by pic_bin_sort on the
 // spline's knot lattice (Node / Cell): the walk, staging, per-(bin,
-// component) register sums, FP64 node gather and stale list of the APIC
+// component) register sums, node gather and stale list of the APIC
 // brick kernel (pic_brick.hpp), with this file's per-particle formulas
 // factored per dimension.  Four work-items per bin, one per grid component
 // (NG <= 3; the fourth idles).  Staged per particle: (f0, f1, f2, v0) and,
