@@ -88,10 +88,10 @@ struct P2GShape {
     // f0..f2 (rows 0-2, written from registers), m (row ROW_M) and the 12
     // scalars of c_p in storage order (rows ROW_C..ROW_C+11).
     static constexpr bool DMA = sizeof(T) == 4;
-    // FP32 quadratic: the staged v_d rows are replaced, once per particle,
-    // by U_d = v_d - h D^-1 sum_a B_da f_a, which leaves the accumulation
-    // loop fewer vector instructions per particle (p2g_component_pre).
-    static constexpr bool PRE = DMA && ORDER == 2;
+    // Quadratic: the staged v_d are replaced, once per particle, by
+    // U_d = v_d - h D^-1 sum_a B_da f_a, which leaves the accumulation loop
+    // fewer vector instructions per particle (p2g_component_pre).
+    static constexpr bool PRE = ORDER == 2;
     static constexpr int ROW_M = 3, ROW_C = 4;
     static constexpr int STAGE = CAP * 16;                   // scalars
     static constexpr int SUMS = CPR * SLOTS;
@@ -162,20 +162,20 @@ __device__ __forceinline__ void p2g_component(const Spline<ORDER, T>& sd, T m, T
 // With d_a = h (o_a - f_a), o = (i, j, k) - 1, the APIC sum
 // m w_i w_j w_k (v + D^-1 (B0 d0 + B1 d1 + B2 d2)) (Picasso_APIC.hpp:110-138)
 // is m w_i w_j w_k (U + h D^-1 (B0 o_0 + B1 o_1 + B2 o_2)).
-__device__ __forceinline__ void p2g_component_pre(const float wm[3], const float w1[3],
-                                                  const float w2[3], float U, float B0, float B1,
-                                                  float B2, float hDinv, float* acc)
+template <typename T>
+__device__ __forceinline__ void p2g_component_pre(const T wm[3], const T w1[3], const T w2[3],
+                                                  T U, T B0, T B1, T B2, T hDinv, T* acc)
 {
-    const float e0 = hDinv * B0, e1 = hDinv * B1, e2 = hDinv * B2;
-    const float ai[3] = {U - e0, U, U + e0};
-    const float wck0 = -(w2[0] * e2), wck2 = w2[2] * e2;
+    const T e0 = hDinv * B0, e1 = hDinv * B1, e2 = hDinv * B2;
+    const T ai[3] = {U - e0, U, U + e0};
+    const T wck0 = -(w2[0] * e2), wck2 = w2[2] * e2;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            const float P = wm[i] * w1[j];
-            const float Tij = j == 0 ? ai[i] - e1 : (j == 1 ? ai[i] : ai[i] + e1);
-            float* a = acc + (i * 3 + j) * 3;
+            const T P = wm[i] * w1[j];
+            const T Tij = j == 0 ? ai[i] - e1 : (j == 1 ? ai[i] : ai[i] + e1);
+            T* a = acc + (i * 3 + j) * 3;
             a[0] += P * (w2[0] * Tij + wck0);
             a[1] += P * (w2[1] * Tij);
             a[2] += P * (w2[2] * Tij + wck2);
@@ -316,6 +316,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
             }
             v.w = a.m.at(a.m.base(p), 0);
             *reinterpret_cast<T4*>(lds + stage_at(q, 0)) = v;
+            const T f3[3] = {v.x, v.y, v.z};
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 // B(d, a) = c_p(1 + a, d) (Picasso_APIC.hpp:94-101).
@@ -323,6 +324,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
                 v.y = a.c.at(cb, 3 + d);
                 v.z = a.c.at(cb, 6 + d);
                 v.w = a.c.at(cb, 9 + d);
+                if constexpr (S::PRE) v.x -= hDinv * (v.y * f3[0] + v.z * f3[1] + v.w * f3[2]);
                 *reinterpret_cast<T4*>(lds + stage_at(q, 1 + d)) = v;
             }
             st.cs.template note<BINS>(k, q, kb, rng);
@@ -334,23 +336,32 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
         const uint32_t q0 = rng[l], q1 = rng[BINS + l];
         for (uint32_t q = q0; q < q1; ++q) {
             if constexpr (S::PRE) {
-                const T x0 = lds[q];
-                if (x0 == T(STALE_F)) continue;
+                T4 fm, ub;
+                if constexpr (S::DMA) {
+                    fm.x = lds[q];
+                    if (fm.x == T(STALE_F)) continue;
+                    fm.y = lds[CAP + q];
+                    fm.z = lds[2 * CAP + q];
+                    fm.w = lds[S::ROW_M * CAP + q];
+                    // U_d = row d, B(d, a) = c(1 + a, d) = row 3 + 3a + d
+                    // of the c block.
+                    const T* cr = lds + (S::ROW_C + grp - 1) * CAP + q;
+                    ub.x = cr[0];
+                    ub.y = cr[3 * CAP];
+                    ub.z = cr[6 * CAP];
+                    ub.w = cr[9 * CAP];
+                } else {
+                    fm = *reinterpret_cast<const T4*>(lds + stage_at(int(q), 0));
+                    if (fm.x == T(STALE_F)) continue;
+                    ub = *reinterpret_cast<const T4*>(lds + stage_at(int(q), grp));
+                }
                 T wm[3], w1[3], w2[3];
-                bspline_w<2, T>(x0, wm);
-                bspline_w<2, T>(lds[CAP + q], w1);
-                bspline_w<2, T>(lds[2 * CAP + q], w2);
-                const T mm = lds[S::ROW_M * CAP + q];
+                bspline_w<2, T>(fm.x, wm);
+                bspline_w<2, T>(fm.y, w1);
+                bspline_w<2, T>(fm.z, w2);
 #pragma unroll
-                for (int i = 0; i < 3; ++i) wm[i] *= mm;
-                // U_d = row d, B(d, a) = c(1 + a, d) = row 3 + 3a + d of
-                // the c block; the mass work-item uses U = 1, B = 0.
-                const T* cr = lds + (S::ROW_C + grp - 1) * CAP + q;
-                T4 ub;
-                ub.x = cr[0];
-                ub.y = cr[3 * CAP];
-                ub.z = cr[6 * CAP];
-                ub.w = cr[9 * CAP];
+                for (int i = 0; i < 3; ++i) wm[i] *= fm.w;
+                // The mass work-item uses U = 1, B = 0.
                 if (comp == 0) {
                     ub.x = T(1);
                     ub.y = ub.z = ub.w = T(0);
