@@ -92,6 +92,11 @@ struct P2GShape {
     // U_d = v_d - h D^-1 sum_a B_da f_a, which leaves the accumulation loop
     // fewer vector instructions per particle (p2g_component_pre).
     static constexpr bool PRE = ORDER == 2;
+    // FP64 quadratic (staged from registers) also folds m / 8 and h D^-1
+    // into the staged scalars and stages g = f + 1/2 for quad_weights2:
+    // group 0 holds (g0, g1, g2, m / 8), group 1 + d holds m U_d / 8 and
+    // m h D^-1 B_da / 8.
+    static constexpr bool FOLD = PRE && !DMA;
     static constexpr int ROW_M = 3, ROW_C = 4;
     static constexpr int STAGE = CAP * 16;                   // scalars
     static constexpr int SUMS = CPR * SLOTS;
@@ -162,11 +167,25 @@ __device__ __forceinline__ void p2g_component(const Spline<ORDER, T>& sd, T m, T
 // With d_a = h (o_a - f_a), o = (i, j, k) - 1, the APIC sum
 // m w_i w_j w_k (v + D^-1 (B0 d0 + B1 d1 + B2 d2)) (Picasso_APIC.hpp:110-138)
 // is m w_i w_j w_k (U + h D^-1 (B0 o_0 + B1 o_1 + B2 o_2)).
+// Twice the quadratic B-spline weights from g = f + 1/2 in [0, 1):
+// w_0 = (1 - g)^2 / 2, w_2 = g^2 / 2 and w_1 = 1 - w_0 - w_2.
 template <typename T>
+__device__ __forceinline__ void quad_weights2(T g, T W[3])
+{
+    const T a = T(1) - g;
+    W[0] = a * a;
+    W[2] = g * g;
+    W[1] = (T(2) - W[0]) - W[2];
+}
+
+// With SCALED, B0..B2 arrive already multiplied by h D^-1 (and wm, U, B by
+// any common factor).
+template <bool SCALED = false, typename T>
 __device__ __forceinline__ void p2g_component_pre(const T wm[3], const T w1[3], const T w2[3],
                                                   T U, T B0, T B1, T B2, T hDinv, T* acc)
 {
-    const T e0 = hDinv * B0, e1 = hDinv * B1, e2 = hDinv * B2;
+    const T e0 = SCALED ? B0 : hDinv * B0, e1 = SCALED ? B1 : hDinv * B1,
+            e2 = SCALED ? B2 : hDinv * B2;
     const T ai[3] = {U - e0, U, U + e0};
     const T wck0 = -(w2[0] * e2), wck2 = w2[2] * e2;
 #pragma unroll
@@ -301,6 +320,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
             const int eb[3] = {f0[0] + lb / (PB * PB), f0[1] + (lb / PB) % PB, f0[2] + lb % PB};
             T4 v;
             T* fr = &v.x;
+            T f3[3];
             bool ok = true;
 #pragma unroll
             for (int dim = 0; dim < 3; ++dim) {
@@ -308,15 +328,16 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
                 const double kb0 = knot_base<ORDER>(double(a.x.at(xb, dim)), a.lat.o[dim],
                                                      a.lat.rdx, fd);
                 ok = ok && int(kb0) + Knots<ORDER>::FIRST == eb[dim];
-                fr[dim] = T(fd);
+                f3[dim] = T(fd);
+                fr[dim] = S::FOLD ? T(fd + 0.5) : T(fd);
             }
             if (!ok) {
                 push_stale(g, s);
                 v.x = T(STALE_F);
             }
-            v.w = a.m.at(a.m.base(p), 0);
+            const T m8 = S::FOLD ? T(0.125) * a.m.at(a.m.base(p), 0) : T(1);
+            v.w = S::FOLD ? m8 : a.m.at(a.m.base(p), 0);
             *reinterpret_cast<T4*>(lds + stage_at(q, 0)) = v;
-            const T f3[3] = {v.x, v.y, v.z};
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 // B(d, a) = c_p(1 + a, d) (Picasso_APIC.hpp:94-101).
@@ -325,6 +346,13 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
                 v.z = a.c.at(cb, 6 + d);
                 v.w = a.c.at(cb, 9 + d);
                 if constexpr (S::PRE) v.x -= hDinv * (v.y * f3[0] + v.z * f3[1] + v.w * f3[2]);
+                if constexpr (S::FOLD) {
+                    const T mh = m8 * hDinv;
+                    v.x *= m8;
+                    v.y *= mh;
+                    v.z *= mh;
+                    v.w *= mh;
+                }
                 *reinterpret_cast<T4*>(lds + stage_at(q, 1 + d)) = v;
             }
             st.cs.template note<BINS>(k, q, kb, rng);
@@ -356,6 +384,18 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
                     ub = *reinterpret_cast<const T4*>(lds + stage_at(int(q), grp));
                 }
                 T wm[3], w1[3], w2[3];
+                if constexpr (S::FOLD) {
+                    quad_weights2(fm.x, wm);
+                    quad_weights2(fm.y, w1);
+                    quad_weights2(fm.z, w2);
+                    // The mass work-item uses U = m / 8, B = 0.
+                    if (comp == 0) {
+                        ub.x = fm.w;
+                        ub.y = ub.z = ub.w = T(0);
+                    }
+                    p2g_component_pre<true>(wm, w1, w2, ub.x, ub.y, ub.z, ub.w, hDinv, acc);
+                    continue;
+                }
                 bspline_w<2, T>(fm.x, wm);
                 bspline_w<2, T>(fm.y, w1);
                 bspline_w<2, T>(fm.z, w2);
