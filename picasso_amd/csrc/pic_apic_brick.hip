@@ -14,9 +14,9 @@
 //   in registers, without any cross-lane or LDS traffic.  The bin sums are
 //   then stored to LDS (plain stores, one slot per bin and stencil offset),
 //   and each work-item of a node of the footprint gathers the up-to-NK^3
-//   sums that land on it in the storage precision (FP64 sums here cost 4 %
-//   of the FP32 kernel for an error far below FP32's own accumulation), then
-//   adds the node total into HBM with one atomic per component.
+//   sums that land on it, adding them in FP64 (exact enough to stand in for
+//   compensated FP32 summation, and an FP64 add costs one FP32 issue slot on
+//   CDNA4), then adds the node total into HBM with one atomic per component.
 //   Nodes shared with neighbouring bricks receive one partial per brick.
 // g2p: the footprint of grid values is staged in LDS with row loads (from
 //   the velocity, or from momentum / mass when normalisation is fused in);

@@ -278,8 +278,9 @@ __device__ __forceinline__ int stage_at(int q, int g)
 // Node gather of one round of bin sums: sums[slot * CPR + c] with slot =
 // offset * BINS + bin, offsets (oi, oj, ok) in [0, NK)^3 and bins of a PB^3
 // brick whose first knot is f0.  Each footprint node adds the (at most NK^3)
-// sums landing on it in the storage precision and applies one atomic per
-// component: component r_lo + cc
+// sums landing on it in FP64, which stands in for compensated FP32
+// summation at less cost than a two-sum (an FP64 add issues at the FP32
+// rate), and applies one atomic per component: component r_lo + cc
 // goes to gm (cc index 0 of round 0, when gm is not null) or to component
 // (r_lo + cc - c_mom) of gmu; components from nwrite on are not written.
 template <int NK, int PB, int NT, int CPR, typename T>
@@ -292,9 +293,9 @@ __device__ __forceinline__ void gather_flush(const T* sums, const int f0[3], int
     constexpr int BINS = PB * PB * PB;
     for (int q = threadIdx.x; q < FV; q += NT) {
         const int fi = q / (FN * FN), fj = (q / FN) % FN, fk = q % FN;
-        T tot[CPR];
+        double tot[CPR];
 #pragma unroll
-        for (int cc = 0; cc < CPR; ++cc) tot[cc] = T(0);
+        for (int cc = 0; cc < CPR; ++cc) tot[cc] = 0.0;
 #pragma unroll 1
         for (int oi = 0; oi < NK; ++oi)
 #pragma unroll
@@ -306,11 +307,11 @@ __device__ __forceinline__ void gather_flush(const T* sums, const int f0[3], int
                     const int slot = ((oi * NK + oj) * NK + ok) * BINS + (bi * PB + bj) * PB + bk;
                     const SumSlot<T, CPR> v = *reinterpret_cast<const SumSlot<T, CPR>*>(sums + slot * CPR);
 #pragma unroll
-                    for (int cc = 0; cc < CPR; ++cc) tot[cc] += v.c[cc];
+                    for (int cc = 0; cc < CPR; ++cc) tot[cc] += double(v.c[cc]);
                 }
         bool any = false;
 #pragma unroll
-        for (int cc = 0; cc < CPR; ++cc) any = any || tot[cc] != T(0);
+        for (int cc = 0; cc < CPR; ++cc) any = any || tot[cc] != 0.0;
         if (!any) continue;
         const int ni = f0[0] + fi, nj = f0[1] + fj, nk = f0[2] + fk;
         if (!(gmu.covers(0, ni, ni) && gmu.covers(1, nj, nj) && gmu.covers(2, nk, nk))) {
@@ -321,7 +322,7 @@ __device__ __forceinline__ void gather_flush(const T* sums, const int f0[3], int
 #pragma unroll
         for (int cc = 0; cc < CPR; ++cc) {
             const int c = r_lo + cc;
-            const T v = tot[cc];
+            const T v = T(tot[cc]);
             if (v == T(0) || c >= nwrite) continue;
             if (c < c_mom) atomic_add(gm.data + gm.off(ni, nj, nk), v);
             else atomic_add(gmu.data + off + (c - c_mom) * gmu.cs, v);
