@@ -247,6 +247,24 @@ inline int64_t initializeParticles(pic_context* ctx, InitRandom, int particles_p
     return n;
 }
 
+// Stable counting sort by bin key (pic_bin_sort); returns the pic_bins that
+// createSpline takes to select the brick kernels.  keys and perm hold np
+// words, bin_start and active pic_bin_count(&geom) + 1.
+inline pic_bins binSort(pic_context* ctx, int dtype, const pic_mesh& mesh, int64_t np,
+                        const pic_pfield& x, const pic_binning& geom, uint32_t* keys,
+                        uint32_t* perm, uint32_t* bin_start, uint32_t* active)
+{
+    check(pic_bin_sort(ctx, dtype, &mesh, np, &x, &geom, keys, perm, bin_start, active));
+    return pic_bins{perm, keys, bin_start, active, geom};
+}
+
+// dst(s, c) = src(perm[s], c) for each of nfields fields (pic_permute).
+inline void permute(pic_context* ctx, int dtype, int64_t np, const uint32_t* perm, int nfields,
+                    const pic_pfield* src, const pic_pfield* dst, const int* ncomp)
+{
+    check(pic_permute(ctx, dtype, np, perm, nfields, src, dst, ncomp));
+}
+
 // kept[0] = number of kept particles, kept[1..] their indices in order.
 inline void select(pic_context* ctx, int64_t np, const uint8_t* keep, uint32_t* kept)
 {

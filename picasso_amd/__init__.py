@@ -30,7 +30,7 @@ __all__ = ["Context", "Mesh", "GridField", "create_spline", "APIC", "PolyPIC", "
            "grid_normalize", "g2p_gradu", "update_F", "bin_sort", "bin_count", "Binning",
            "BinSort", "PicError", "DUAL_DIMS", "aligned_dual", "InitUniform", "InitRandom",
            "initialize_particles", "select", "filter_particles", "halo_scatter",
-           "halo_gather",
+           "halo_gather", "AoSoA", "AoSoASlice", "permute", "G2P", "P2G",
            "lib", "LIB_PATH"]
 
 LIB_PATH = _abi.LIB_PATH
@@ -145,8 +145,10 @@ class Context:
         return StepGraph(self, g)
 
     def close(self):
+        """Destroy the context.  Refused (PicError) while a step graph made
+        from it is alive: close the graphs first."""
         if getattr(self, "handle", None):
-            lib.pic_context_destroy(self.handle)
+            check(lib.pic_context_destroy(self.handle), "pic_context_destroy")
             self.handle = None
 
     def __del__(self):
@@ -158,7 +160,9 @@ class Context:
 
 class StepGraph:
     """A captured sequence of calls (pic_graph); launch() replays it on the
-    context's current stream."""
+    context's current stream.  While it is alive the context's scratch
+    cannot grow: an eager call needing more raises PicError
+    (PIC_ERR_ALLOC)."""
 
     def __init__(self, ctx, handle):
         self.ctx = ctx
@@ -209,14 +213,127 @@ class GridField:
                          st[3] if t.dim() == 4 else 0)
 
 
-def _pfield(t, ncomp, name):
+class AoSoA:
+    """A Cabana AoSoA buffer as Picasso's particle lists hold it
+    (Picasso_ParticleList.hpp:90-138): tiles of V particles; inside a tile
+    every member's scalars are stored component-major, V consecutive
+    particles per component.  `buffer` is a 1-D device tensor of the
+    scalars of all tiles, `members` the scalar count of each member in
+    order (e.g. (3, 1, 12) for x, m, c_p)."""
+
+    def __init__(self, buffer, vector_length, members, size):
+        v = int(vector_length)
+        if v < 1 or v & (v - 1):
+            raise ValueError("vector_length must be a power of two")
+        self.buffer, self.V, self.members, self.size = buffer, v, tuple(members), int(size)
+        self.q = sum(self.members)
+        need = -(-self.size // v) * v * self.q
+        if buffer.dim() != 1 or buffer.shape[0] < need:
+            raise ValueError(f"AoSoA buffer must be 1-D with at least {need} scalars")
+
+    def slice(self, member):
+        """The particle field of one member (Cabana::slice<member>)."""
+        return AoSoASlice(self, sum(self.members[:member]), self.members[member])
+
+    @staticmethod
+    def from_fields(fields, vector_length):
+        """Pack [np, q_f] tensors (one per member, same dtype and device)
+        into a new AoSoA."""
+        import torch
+        np_ = fields[0].shape[0]
+        v = int(vector_length)
+        q = sum(f.shape[1] for f in fields)
+        nt = -(-np_ // v)
+        buf = torch.zeros(nt * v * q, dtype=fields[0].dtype, device=fields[0].device)
+        tiles = buf.view(nt, q, v)
+        o = 0
+        for f in fields:
+            pad = torch.zeros((nt * v, f.shape[1]), dtype=f.dtype, device=f.device)
+            pad[:np_] = f
+            tiles[:, o:o + f.shape[1], :] = pad.view(nt, v, f.shape[1]).transpose(1, 2)
+            o += f.shape[1]
+        return AoSoA(buf, v, [f.shape[1] for f in fields], np_)
+
+
+class AoSoASlice:
+    """One member of an AoSoA: element (p, c) at buffer[(p // V) * V q +
+    (offset + c) V + p % V], i.e. vlen_log2 = log2 V, tile stride V q and
+    component stride V in the pic_pfield formula."""
+
+    def __init__(self, aosoa, offset, ncomp):
+        self.aosoa, self.offset, self.ncomp = aosoa, offset, ncomp
+
+    @property
+    def shape(self):
+        return (self.aosoa.size, self.ncomp)
+
+    @property
+    def dtype(self):
+        return self.aosoa.buffer.dtype
+
+    @property
+    def is_cuda(self):
+        return self.aosoa.buffer.is_cuda
+
+    @property
+    def device(self):
+        return self.aosoa.buffer.device
+
+    def dim(self):
+        return 2
+
+    def to_tensor(self):
+        """A [size, ncomp] copy of the member (for checks)."""
+        a = self.aosoa
+        nt = -(-a.size // a.V)
+        t = a.buffer[:nt * a.V * a.q].view(nt, a.q, a.V)[:, self.offset:self.offset + self.ncomp, :]
+        return t.transpose(1, 2).reshape(nt * a.V, self.ncomp)[:a.size]
+
+    def c(self):
+        a = self.aosoa
+        esz = a.buffer.element_size()
+        return _abi.PField(a.buffer.data_ptr() + self.offset * a.V * esz,
+                           a.V.bit_length() - 1, a.V * a.q, a.V)
+
+
+def _pfield(t, ncomp, name, np_=None, dtype=None):
+    """pic_pfield of a [np, ncomp] tensor (any strides) or an AoSoA slice,
+    after checking its width, its row count against np_ and its dtype
+    against the call's (every check raises before anything is launched)."""
+    if isinstance(t, AoSoASlice):
+        if t.ncomp != ncomp:
+            raise ValueError(f"{name} must have {ncomp} components, got {t.ncomp}")
+    else:
+        if t.dim() == 1:
+            t = t.unsqueeze(1)
+        if t.dim() != 2 or t.shape[1] != ncomp:
+            raise ValueError(f"{name} must be [np, {ncomp}], got {tuple(t.shape)}")
+    if np_ is not None and t.shape[0] < np_:
+        raise ValueError(f"{name} holds {t.shape[0]} particles, fewer than np = {np_}")
+    if dtype is not None and _dtype_code(t) != dtype:
+        raise TypeError(f"{name} is {t.dtype}; every field of the call must share x's dtype")
     _require_device(t, name)
-    if t.dim() == 1:
-        t = t.unsqueeze(1)
-    if t.dim() != 2 or t.shape[1] != ncomp:
-        raise ValueError(f"{name} must be [np, {ncomp}], got {tuple(t.shape)}")
+    if isinstance(t, AoSoASlice):
+        return t.c()
     st = t.stride()
     return _abi.PField(t.data_ptr(), 0, st[0], st[1])
+
+
+def _grid(g, name, dtype, ncomp=None):
+    """pic_grid of a GridField after checking its dtype and, when given, its
+    component count."""
+    if _dtype_code(g.tensor) != dtype:
+        raise TypeError(f"{name} is {g.tensor.dtype}; every field of the call must share "
+                        "x's dtype")
+    if ncomp is not None and g.ncomp != ncomp:
+        raise ValueError(f"{name} must have {ncomp} component(s), got {g.ncomp}")
+    return g.c()
+
+
+def _ncomp_grid(entity, collocated):
+    """Components of a transfer's momentum / velocity grid: `collocated`
+    on Node/Cell, 1 on a staggered Face/Edge lattice."""
+    return collocated if entity in (NODE, CELL) else 1
 
 
 @dataclass
@@ -239,8 +356,12 @@ def create_spline(location, order, mesh, x, bins=None):
     return Spline(location, order, mesh, x, bins)
 
 
-def _bins_ptr(bins):
-    return ctypes.byref(bins.c()) if bins is not None else None
+def _bins_ptr(bins, np_):
+    """pic_bins of a BinSort after checking that it covers np_ particles."""
+    if bins is None:
+        return None
+    _check_bins(bins, np_)
+    return ctypes.byref(bins.c())
 
 
 class APIC:
@@ -249,40 +370,39 @@ class APIC:
         """APIC::p2g(m_p, c_p, m_i, mu_i, sd) (Picasso_APIC.hpp:55-402).
         c_p is [np, 12] (row-major 4x3); accumulates into m_i, mu_i."""
         x = sd.x
-        np_ = x.shape[0]
-        mesh = sd.mesh.c()
-        rc = lib.pic_apic_p2g(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
-                              ctypes.byref(mesh), np_, ctypes.byref(_pfield(x, 3, "x")),
-                              ctypes.byref(_pfield(m_p, 1, "m_p")),
-                              ctypes.byref(_pfield(c_p, 12, "c_p")), ctypes.byref(m_i.c()),
-                              ctypes.byref(mu_i.c()), _bins_ptr(sd.bins),
-                              _abi.P2G_ZERO_FIRST if zero_first else 0)
-        check(rc, "pic_apic_p2g")
+        np_, dt = x.shape[0], _dtype_code(x)
+        args = (ctypes.byref(sd.mesh.c()), np_, ctypes.byref(_pfield(x, 3, "x")),
+                ctypes.byref(_pfield(m_p, 1, "m_p", np_, dt)),
+                ctypes.byref(_pfield(c_p, 12, "c_p", np_, dt)),
+                ctypes.byref(_grid(m_i, "m_i", dt, 1)),
+                ctypes.byref(_grid(mu_i, "mu_i", dt, _ncomp_grid(sd.location, 3))),
+                _bins_ptr(sd.bins, np_), _abi.P2G_ZERO_FIRST if zero_first else 0)
+        check(lib.pic_apic_p2g(ctx._bind_stream(), sd.order, sd.location, dt, *args),
+              "pic_apic_p2g")
 
     @staticmethod
     def g2p(ctx, u_i, c_p, sd):
         """APIC::g2p(u_i, c_p, sd) (Picasso_APIC.hpp:407-544)."""
         x = sd.x
-        mesh = sd.mesh.c()
-        rc = lib.pic_apic_g2p(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
-                              ctypes.byref(mesh), x.shape[0], ctypes.byref(_pfield(x, 3, "x")),
-                              ctypes.byref(u_i.c()), ctypes.byref(_pfield(c_p, 12, "c_p")),
-                              _bins_ptr(sd.bins))
-        check(rc, "pic_apic_g2p")
+        np_, dt = x.shape[0], _dtype_code(x)
+        args = (ctypes.byref(sd.mesh.c()), np_, ctypes.byref(_pfield(x, 3, "x")),
+                ctypes.byref(_grid(u_i, "u_i", dt, _ncomp_grid(sd.location, 3))),
+                ctypes.byref(_pfield(c_p, 12, "c_p", np_, dt)), _bins_ptr(sd.bins, np_))
+        check(lib.pic_apic_g2p(ctx._bind_stream(), sd.order, sd.location, dt, *args),
+              "pic_apic_g2p")
 
     @staticmethod
     def g2p_normalized(ctx, m_i, mu_i, c_p, sd):
         """APIC::g2p of u_i = mu_i / m_i computed on the fly (grid
         normalisation fused into g2p)."""
         x = sd.x
-        mesh = sd.mesh.c()
-        rc = lib.pic_apic_g2p_normalized(ctx._bind_stream(), sd.order, sd.location,
-                                         _dtype_code(x), ctypes.byref(mesh), x.shape[0],
-                                         ctypes.byref(_pfield(x, 3, "x")), ctypes.byref(m_i.c()),
-                                         ctypes.byref(mu_i.c()),
-                                         ctypes.byref(_pfield(c_p, 12, "c_p")),
-                                         _bins_ptr(sd.bins))
-        check(rc, "pic_apic_g2p_normalized")
+        np_, dt = x.shape[0], _dtype_code(x)
+        args = (ctypes.byref(sd.mesh.c()), np_, ctypes.byref(_pfield(x, 3, "x")),
+                ctypes.byref(_grid(m_i, "m_i", dt, 1)),
+                ctypes.byref(_grid(mu_i, "mu_i", dt, _ncomp_grid(sd.location, 3))),
+                ctypes.byref(_pfield(c_p, 12, "c_p", np_, dt)), _bins_ptr(sd.bins, np_))
+        check(lib.pic_apic_g2p_normalized(ctx._bind_stream(), sd.order, sd.location, dt, *args),
+              "pic_apic_g2p_normalized")
 
     @staticmethod
     def g2p_update_F(ctx, u_i, c_p, dt, F, det, Finv, sd, m_i=None):
@@ -291,23 +411,17 @@ class APIC:
         With m_i given, u_i is the grid momentum and the velocity mu/m is
         formed on the fly.  F [np, 9] (updated in place), det [np, 1] and
         Finv [np, 9] are float64."""
-        import torch
         x = sd.x
-        mesh = sd.mesh.c()
-        for t, nm in ((F, "F"), (det, "det"), (Finv, "Finv")):
-            if t.dtype != torch.float64:
-                raise TypeError(f"{nm} must be float64")
-        rc = lib.pic_apic_g2p_update_F(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
-                                       ctypes.byref(mesh), x.shape[0],
-                                       ctypes.byref(_pfield(x, 3, "x")),
-                                       ctypes.byref(m_i.c()) if m_i is not None else None,
-                                       ctypes.byref(u_i.c()),
-                                       ctypes.byref(_pfield(c_p, 12, "c_p")), float(dt),
-                                       ctypes.byref(_pfield(F, 9, "F")),
-                                       ctypes.byref(_pfield(det, 1, "det")),
-                                       ctypes.byref(_pfield(Finv, 9, "Finv")),
-                                       _bins_ptr(sd.bins))
-        check(rc, "pic_apic_g2p_update_F")
+        np_, dty = x.shape[0], _dtype_code(x)
+        args = (ctypes.byref(sd.mesh.c()), np_, ctypes.byref(_pfield(x, 3, "x")),
+                ctypes.byref(_grid(m_i, "m_i", dty, 1)) if m_i is not None else None,
+                ctypes.byref(_grid(u_i, "u_i", dty, 3)),
+                ctypes.byref(_pfield(c_p, 12, "c_p", np_, dty)), float(dt),
+                ctypes.byref(_pfield(F, 9, "F", np_, F64)),
+                ctypes.byref(_pfield(det, 1, "det", np_, F64)),
+                ctypes.byref(_pfield(Finv, 9, "Finv", np_, F64)), _bins_ptr(sd.bins, np_))
+        check(lib.pic_apic_g2p_update_F(ctx._bind_stream(), sd.order, sd.location, dty, *args),
+              "pic_apic_g2p_update_F")
 
 
 class PolyPIC:
@@ -321,38 +435,38 @@ class PolyPIC:
         (Picasso_PolyPIC.hpp:39-229).  u_p is [np, nmode*3], c_p
         [np, nmode*ncomp]."""
         x = sd.x
+        np_, dty = x.shape[0], _dtype_code(x)
         nm = PolyPIC.nmode(sd.order)
-        mesh = sd.mesh.c()
-        rc = lib.pic_polypic_p2g(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
-                                 ctypes.byref(mesh), x.shape[0], ncomp,
-                                 ctypes.byref(_pfield(x, 3, "x")),
-                                 ctypes.byref(_pfield(m_p, 1, "m_p")),
-                                 ctypes.byref(_pfield(u_p, nm * 3, "u_p")),
-                                 ctypes.byref(_pfield(c_p, nm * ncomp, "c_p")),
-                                 ctypes.byref(mu_i.c()), ctypes.byref(m_i.c()), float(dt),
-                                 _bins_ptr(sd.bins), _abi.P2G_ZERO_FIRST if zero_first else 0)
-        check(rc, "pic_polypic_p2g")
+        args = (ctypes.byref(sd.mesh.c()), np_, ncomp, ctypes.byref(_pfield(x, 3, "x")),
+                ctypes.byref(_pfield(m_p, 1, "m_p", np_, dty)),
+                ctypes.byref(_pfield(u_p, nm * 3, "u_p", np_, dty)),
+                ctypes.byref(_pfield(c_p, nm * ncomp, "c_p", np_, dty)),
+                ctypes.byref(_grid(mu_i, "mu_i", dty, _ncomp_grid(sd.location, ncomp))),
+                ctypes.byref(_grid(m_i, "m_i", dty, 1)), float(dt), _bins_ptr(sd.bins, np_),
+                _abi.P2G_ZERO_FIRST if zero_first else 0)
+        check(lib.pic_polypic_p2g(ctx._bind_stream(), sd.order, sd.location, dty, *args),
+              "pic_polypic_p2g")
 
     @staticmethod
     def g2p(ctx, u_i, c_p, sd, ncomp=3):
         """PolyPIC::g2p(u_i, c_p, sd) (Picasso_PolyPIC.hpp:235-359)."""
         x = sd.x
+        np_, dty = x.shape[0], _dtype_code(x)
         nm = PolyPIC.nmode(sd.order)
-        mesh = sd.mesh.c()
-        rc = lib.pic_polypic_g2p(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
-                                 ctypes.byref(mesh), x.shape[0], ncomp,
-                                 ctypes.byref(_pfield(x, 3, "x")), ctypes.byref(u_i.c()),
-                                 ctypes.byref(_pfield(c_p, nm * ncomp, "c_p")),
-                                 _bins_ptr(sd.bins))
-        check(rc, "pic_polypic_g2p")
+        args = (ctypes.byref(sd.mesh.c()), np_, ncomp, ctypes.byref(_pfield(x, 3, "x")),
+                ctypes.byref(_grid(u_i, "u_i", dty, _ncomp_grid(sd.location, ncomp))),
+                ctypes.byref(_pfield(c_p, nm * ncomp, "c_p", np_, dty)), _bins_ptr(sd.bins, np_))
+        check(lib.pic_polypic_g2p(ctx._bind_stream(), sd.order, sd.location, dty, *args),
+              "pic_polypic_g2p")
 
 
 def grid_normalize(ctx, m_i, mu_i, u_i, rel_mass_eps=0.0):
     """u_i = mu_i / m_i (0 where m_i == 0)."""
-    rc = lib.pic_grid_normalize(ctx._bind_stream(), _dtype_code(m_i.tensor), mu_i.ncomp,
-                                ctypes.byref(m_i.c()), ctypes.byref(mu_i.c()),
-                                ctypes.byref(u_i.c()), float(rel_mass_eps))
-    check(rc, "pic_grid_normalize")
+    dt = _dtype_code(m_i.tensor)
+    args = (mu_i.ncomp, ctypes.byref(_grid(m_i, "m_i", dt, 1)),
+            ctypes.byref(_grid(mu_i, "mu_i", dt)),
+            ctypes.byref(_grid(u_i, "u_i", dt, mu_i.ncomp)), float(rel_mass_eps))
+    check(lib.pic_grid_normalize(ctx._bind_stream(), dt, *args), "pic_grid_normalize")
 
 
 HALO_GATHER, HALO_SCATTER_SUM = 0, 1
@@ -382,12 +496,12 @@ def halo_gather(ctx, field, own_lo, own_n, periodic=(True, True, True)):
 def g2p_gradu(ctx, u_i, gradu, sd):
     """Velocity gradient gradu[p, i*3+j] = du_i/dx_j (G2P::gradient)."""
     x = sd.x
-    mesh = sd.mesh.c()
-    rc = lib.pic_g2p_gradu(ctx._bind_stream(), sd.order, sd.location, _dtype_code(x),
-                           ctypes.byref(mesh), x.shape[0], ctypes.byref(_pfield(x, 3, "x")),
-                           ctypes.byref(u_i.c()), ctypes.byref(_pfield(gradu, 9, "gradu")),
-                           _bins_ptr(sd.bins))
-    check(rc, "pic_g2p_gradu")
+    np_, dt = x.shape[0], _dtype_code(x)
+    args = (ctypes.byref(sd.mesh.c()), np_, ctypes.byref(_pfield(x, 3, "x")),
+            ctypes.byref(_grid(u_i, "u_i", dt, 3)),
+            ctypes.byref(_pfield(gradu, 9, "gradu", np_, dt)), _bins_ptr(sd.bins, np_))
+    check(lib.pic_g2p_gradu(ctx._bind_stream(), sd.order, sd.location, dt, *args),
+          "pic_g2p_gradu")
 
 
 INTERP_VALUE, INTERP_GRADIENT, INTERP_DIVERGENCE = 0, 1, 2
@@ -395,23 +509,25 @@ INTERP_VALUE, INTERP_GRADIENT, INTERP_DIVERGENCE = 0, 1, 2
 
 def _g2p_interp(ctx, op, sd, field, ncomp, result, nout, what):
     x = sd.x
-    mesh = sd.mesh.c()
-    rc = lib.pic_g2p_interp(ctx._bind_stream(), op, sd.order, sd.location, _dtype_code(x),
-                            ctypes.byref(mesh), x.shape[0], ctypes.byref(_pfield(x, 3, "x")),
-                            ctypes.byref(field.c()), ncomp,
-                            ctypes.byref(_pfield(result, nout, "result")), _bins_ptr(sd.bins))
-    check(rc, what)
+    np_, dt = x.shape[0], _dtype_code(x)
+    args = (ctypes.byref(sd.mesh.c()), np_, ctypes.byref(_pfield(x, 3, "x")),
+            ctypes.byref(_grid(field, "field", dt, ncomp)), ncomp,
+            ctypes.byref(_pfield(result, nout, "result", np_, dt)), _bins_ptr(sd.bins, np_))
+    check(lib.pic_g2p_interp(ctx._bind_stream(), op, sd.order, sd.location, dt, *args), what)
+
+
+# Grid components a P2G writes, per (operation, value components).
+_P2G_FIELD_NCOMP = {(0, 1): 1, (0, 2): 2, (0, 3): 3, (1, 1): 3, (2, 3): 1, (2, 9): 3}
 
 
 def _p2g_interp(ctx, op, sd, value, ncomp, field, zero_first, what):
     x = sd.x
-    mesh = sd.mesh.c()
-    rc = lib.pic_p2g_interp(ctx._bind_stream(), op, sd.order, sd.location, _dtype_code(x),
-                            ctypes.byref(mesh), x.shape[0], ctypes.byref(_pfield(x, 3, "x")),
-                            ctypes.byref(_pfield(value, ncomp, "value")), ncomp,
-                            ctypes.byref(field.c()),
-                            _abi.P2G_ZERO_FIRST if zero_first else 0, _bins_ptr(sd.bins))
-    check(rc, what)
+    np_, dt = x.shape[0], _dtype_code(x)
+    args = (ctypes.byref(sd.mesh.c()), np_, ctypes.byref(_pfield(x, 3, "x")),
+            ctypes.byref(_pfield(value, ncomp, "value", np_, dt)), ncomp,
+            ctypes.byref(_grid(field, "field", dt, _P2G_FIELD_NCOMP.get((op, ncomp)))),
+            _abi.P2G_ZERO_FIRST if zero_first else 0, _bins_ptr(sd.bins, np_))
+    check(lib.pic_p2g_interp(ctx._bind_stream(), op, sd.order, sd.location, dt, *args), what)
 
 
 class G2P:
@@ -465,11 +581,12 @@ class P2G:
 
 def update_F(ctx, gradu, dt, F, det, Finv):
     """F <- (I + dt gradu) F, det(F), F^{-1} in FP64."""
-    rc = lib.pic_update_F(ctx._bind_stream(), F.shape[0], _dtype_code(gradu),
-                          ctypes.byref(_pfield(gradu, 9, "gradu")), float(dt),
-                          ctypes.byref(_pfield(F, 9, "F")), ctypes.byref(_pfield(det, 1, "det")),
-                          ctypes.byref(_pfield(Finv, 9, "Finv")))
-    check(rc, "pic_update_F")
+    np_ = F.shape[0]
+    args = (_dtype_code(gradu), ctypes.byref(_pfield(gradu, 9, "gradu", np_)), float(dt),
+            ctypes.byref(_pfield(F, 9, "F", np_, F64)),
+            ctypes.byref(_pfield(det, 1, "det", np_, F64)),
+            ctypes.byref(_pfield(Finv, 9, "Finv", np_, F64)))
+    check(lib.pic_update_F(ctx._bind_stream(), np_, *args), "pic_update_F")
 
 
 @dataclass
@@ -605,11 +722,30 @@ def bin_sort(ctx, mesh, x, geom, out=None, with_active=True):
         out = BinSort(torch.empty(np_, **i32), torch.empty(np_, **i32),
                       torch.empty(nb + 1, **i32), geom,
                       torch.empty(nb + 1, **i32) if with_active else None)
-    m = mesh.c()
-    g = geom.c()
-    rc = lib.pic_bin_sort(ctx._bind_stream(), _dtype_code(x), ctypes.byref(m), np_,
-                          ctypes.byref(_pfield(x, 3, "x")), ctypes.byref(g),
-                          out.keys.data_ptr(), out.perm.data_ptr(), out.bin_start.data_ptr(),
-                          out.active.data_ptr() if out.active is not None else None)
-    check(rc, "pic_bin_sort")
+    _check_bins(out, np_, nb)
+    args = (ctypes.byref(mesh.c()), np_, ctypes.byref(_pfield(x, 3, "x")),
+            ctypes.byref(geom.c()), out.keys.data_ptr(), out.perm.data_ptr(),
+            out.bin_start.data_ptr(), out.active.data_ptr() if out.active is not None else None)
+    check(lib.pic_bin_sort(ctx._bind_stream(), _dtype_code(x), *args), "pic_bin_sort")
     return out
+
+
+def _check_bins(bins, np_, nb=None):
+    """A BinSort must hold np_ keys and slots and nb + 1 brick offsets (and
+    active entries); raises ValueError before anything is launched."""
+    import torch
+    if nb is None:
+        nb = bin_count(bins.geom)
+    parts = [(name, t, need) for name, t, need in
+             (("perm", bins.perm, np_), ("keys", bins.keys, np_),
+              ("bin_start", bins.bin_start, nb + 1), ("active", bins.active, nb + 1))
+             if t is not None or name != "active"]
+    for name, t, need in parts:
+        if t is None:
+            raise ValueError(f"bins.{name} is missing")
+        if t.dtype not in (torch.int32, torch.uint32) or t.dim() != 1 or not t.is_contiguous():
+            raise TypeError(f"bins.{name} must be a contiguous 1-D int32 tensor")
+        if t.shape[0] < need:
+            raise ValueError(f"bins.{name} holds {t.shape[0]} entries, fewer than {need}")
+    for name, t, _ in parts:
+        _require_device(t, f"bins.{name}")

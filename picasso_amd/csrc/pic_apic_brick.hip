@@ -536,6 +536,13 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const Brick
                 push_stale(g, s);
                 continue;
             }
+            // Footprint nodes outside the window were staged as zeros: a
+            // stencil reaching them is an error, as in the per-particle
+            // gather.
+            if (!stencil_inside(sd, gu)) {
+                atomicOr(a.err, ERRBIT_BOUNDS);
+                continue;
+            }
             const int lbase = (li * FN + lj) * FN + lk;
             T cp[12], L[9];
             apic_g2p_particle<ORDER, STAG, GF>(a, sd, [&](int i, int j, int k, T* ue) {
@@ -566,15 +573,16 @@ __global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
 }
 
 template <typename T>
-static bool make_brick_args(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
+static int make_brick_args(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
                             int64_t np, const pic_pfield& x, const pic_pfield* m,
                             const pic_pfield& c, const pic_grid* gm, const pic_grid& gmu,
                             const pic_grid* gmass, const pic_bins& bins, BrickArgs<T>& g)
 {
-    if (!setup_walk(ctx, order, entity, np, bins, g)) return false;
+    const int rc = setup_walk(ctx, order, entity, np, bins, g);
+    if (rc) return rc;
     g.a = make_args<T>(ctx, entity, mesh, np, &x, m, &c, gm, gmu, bins.perm);
     if (gmass) g.a.gnm = make_gview<T>(*gmass);
-    return true;
+    return PIC_OK;
 }
 
 template <typename T>
@@ -641,8 +649,9 @@ static int p2g_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
 {
     if (brick_edge(bins.geom) != PB) return PIC_ERR_UNSUPPORTED;
     BrickArgs<T> g;
-    if (!make_brick_args<T>(ctx, order, entity, mesh, np, x, &m, c, &gm, gmu, nullptr, bins, g))
-        return PIC_ERR_UNSUPPORTED;
+    if (const int rc = make_brick_args<T>(ctx, order, entity, mesh, np, x, &m, c, &gm, gmu, nullptr,
+                                          bins, g))
+        return rc;
     {
         TimedLaunch tl(ctx, "apic_p2g");
         P2GBrickLaunch<T>{ctx, g, stag_dim(entity) >= 0}.by_order(order);
@@ -658,9 +667,9 @@ static int g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
 {
     if (fu && stag_dim(entity) >= 0) return PIC_ERR_UNSUPPORTED;
     BrickArgs<T> g;
-    if (!make_brick_args<T>(ctx, order, entity, mesh, np, x, nullptr, c, nullptr, gu, gmass, bins,
-                            g))
-        return PIC_ERR_UNSUPPORTED;
+    if (const int rc = make_brick_args<T>(ctx, order, entity, mesh, np, x, nullptr, c, nullptr, gu,
+                                          gmass, bins, g))
+        return rc;
     set_F_update(g.a, fu);
     {
         TimedLaunch tl(ctx, "apic_g2p");

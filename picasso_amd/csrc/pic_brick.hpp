@@ -358,17 +358,20 @@ inline int brick_edge(const pic_binning& geom)
     return (e == 4 || e == 8) ? e : 0;
 }
 
-// Fill the walk fields of g for bins; false when the brick kernels do not
-// apply (then the caller falls back to the per-particle kernels).
+// Fill the walk fields of g for bins.  PIC_ERR_UNSUPPORTED when the brick
+// kernels do not apply (the caller then falls back to the per-particle
+// kernels), PIC_ERR_ALLOC when the scratch cannot grow (inside a capture, or
+// while a step graph of the context is alive).
 template <typename G>
-inline bool setup_walk(pic_context* ctx, int order, int entity, int64_t np, const pic_bins& bins,
-                       G& g)
+inline int setup_walk(pic_context* ctx, int order, int entity, int64_t np, const pic_bins& bins,
+                      G& g)
 {
     const int bb = brick_edge(bins.geom);
-    if (!bb || !bins.keys || !bins.bin_start || !bins.perm) return false;
-    if (!aligned_knot_offset(order, entity, bins.geom.dual, g.knot_off)) return false;
+    if (!bb || !bins.keys || !bins.bin_start || !bins.perm) return PIC_ERR_UNSUPPORTED;
+    if (!aligned_knot_offset(order, entity, bins.geom.dual, g.knot_off))
+        return PIC_ERR_UNSUPPORTED;
     const int64_t nb = pic_bin_count(&bins.geom);
-    if (nb <= 0) return false;
+    if (nb <= 0) return PIC_ERR_UNSUPPORTED;
     g.keys = bins.keys;
     g.bin_start = bins.bin_start;
     g.active = bins.active;
@@ -381,7 +384,7 @@ inline bool setup_walk(pic_context* ctx, int order, int entity, int64_t np, cons
     const size_t ibytes = size_t(nb) * sizeof(BrickItem);
     const size_t words = size_t(WALK_LANES) * WALK_PAD + 1 + size_t(np);
     char* base = static_cast<char*>(scratch(ctx, ibytes + words * sizeof(uint32_t)));
-    if (!base) return false;
+    if (!base) return PIC_ERR_ALLOC;
     BrickItem* items = reinterpret_cast<BrickItem*>(base);
     uint32_t* w = reinterpret_cast<uint32_t*>(base + ibytes);
     g.items = items;
@@ -390,7 +393,7 @@ inline bool setup_walk(pic_context* ctx, int order, int entity, int64_t np, cons
     const unsigned nblk = unsigned(std::min<int64_t>((nb + 255) / 256, 2048));
     k_walk_items<<<nblk, 256, 0, ctx->stream>>>(bins.active, bins.bin_start, uint32_t(nb), items,
                                                   w, g.stale);
-    return launch_status() == 0;
+    return launch_status();
 }
 
 }  // namespace pic

@@ -9,7 +9,9 @@ namespace pic {
 void* scratch(pic_context* ctx, size_t bytes)
 {
     if (bytes <= ctx->scratch_bytes) return ctx->scratch;
-    if (ctx->capturing) return nullptr;  // no allocation inside a graph capture
+    // No allocation inside a graph capture, and no reallocation while a
+    // captured graph still points into the buffer.
+    if (ctx->capturing || ctx->live_graphs > 0) return nullptr;
     // The stream may still use the old buffer.
     if (ctx->scratch) {
         if (hipStreamSynchronize(ctx->stream) != hipSuccess) return nullptr;
@@ -179,7 +181,7 @@ int pic_context_create(pic_context** out, int device)
 
 int pic_context_destroy(pic_context* ctx)
 {
-    if (!ctx) return PIC_ERR_ARG;
+    if (!ctx || ctx->live_graphs > 0) return PIC_ERR_ARG;
     DeviceGuard g(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (auto& r : ctx->recs) {
@@ -263,6 +265,7 @@ struct pic_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int device = 0;
+    pic_context* ctx = nullptr;  // whose scratch the captured kernels use
 };
 
 int pic_graph_begin(pic_context* ctx)
@@ -293,13 +296,15 @@ int pic_graph_end(pic_context* ctx, pic_graph** out)
     pg->graph = graph;
     pg->exec = exec;
     pg->device = ctx->device;
+    pg->ctx = ctx;
+    ++ctx->live_graphs;
     *out = pg;
     return PIC_OK;
 }
 
 int pic_graph_launch(pic_context* ctx, const pic_graph* graph)
 {
-    if (!ctx || !graph || ctx->capturing || graph->device != ctx->device) return PIC_ERR_ARG;
+    if (!ctx || !graph || ctx->capturing || graph->ctx != ctx) return PIC_ERR_ARG;
     DeviceGuard g(ctx->device);
     if (!g.ok) return PIC_ERR_HIP;
     PIC_HIP_CHECK(hipGraphLaunch(graph->exec, ctx->stream));
@@ -312,6 +317,7 @@ int pic_graph_destroy(pic_graph* graph)
     DeviceGuard g(graph->device);
     if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
     if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    if (graph->ctx) --graph->ctx->live_graphs;
     delete graph;
     return PIC_OK;
 }

@@ -36,6 +36,10 @@ struct pic_context {
     // Between pic_graph_begin and pic_graph_end: calls are being captured
     // (no scratch growth, no timing events, no synchronisation).
     bool capturing = false;
+    // Step graphs made from this context and not yet destroyed.  They hold
+    // pointers into the scratch buffer, so it may not be reallocated while
+    // any is alive.
+    int live_graphs = 0;
 
     // Resident workgroups per compute unit, per (kernel, dynamic LDS bytes).
     std::map<std::pair<const void*, size_t>, int> occupancy;
@@ -64,7 +68,9 @@ struct DeviceGuard {
     }
 };
 
-// Ensure at least `bytes` of scratch; returns nullptr on failure.
+// Ensure at least `bytes` of scratch; returns nullptr on failure, and when
+// the buffer would have to grow during a capture or while a step graph of
+// the context is alive (its replays use the current buffer).
 void* scratch(pic_context* ctx, size_t bytes);
 
 // Launch bracketing for optional timing.

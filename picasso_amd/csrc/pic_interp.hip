@@ -461,7 +461,7 @@ static int p2g_interp_brick_run(pic_context* ctx, int op, int order, int entity,
 {
     if (brick_edge(bins.geom) != IPB) return PIC_ERR_UNSUPPORTED;
     InterpBrickArgs<T> g;
-    if (!setup_walk(ctx, order, entity, a.np, bins, g)) return PIC_ERR_UNSUPPORTED;
+    if (const int rc = setup_walk(ctx, order, entity, a.np, bins, g)) return rc;
     g.a = a;
     {
         TimedLaunch tl(ctx, "p2g_interp");

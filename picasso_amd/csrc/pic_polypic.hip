@@ -475,6 +475,12 @@ __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
             push_stale(g, s);
             continue;
         }
+        // Footprint nodes outside the window were staged as zeros: a
+        // stencil reaching them is an error, as in the per-particle gather.
+        if (!stencil_inside(sd, gu)) {
+            atomicOr(a.err, ERRBIT_BOUNDS);
+            continue;
+        }
         // F[q][e](n): weight, gradient, half second derivative of dim q.
         T F[3][E][NK];
 #pragma unroll
@@ -652,7 +658,7 @@ static int polypic_p2g_brick_run(pic_context* ctx, int order, int entity, int nc
 {
     if (brick_edge(bins.geom) != 4) return PIC_ERR_UNSUPPORTED;
     PolyBrickArgs<T> g;
-    if (!setup_walk(ctx, order, entity, np, bins, g)) return PIC_ERR_UNSUPPORTED;
+    if (const int rc = setup_walk(ctx, order, entity, np, bins, g)) return rc;
     g.a = make_poly_args<T>(ctx, entity, ncomp, mesh, np, x, m, u, c, gmu, gm, dt, bins.perm);
     const bool stag = stag_dim(entity) >= 0;
     {
@@ -687,7 +693,7 @@ static int polypic_g2p_brick_run(pic_context* ctx, int order, int entity, int nc
 {
     if (brick_edge(bins.geom) != 4) return PIC_ERR_UNSUPPORTED;
     PolyBrickArgs<T> g;
-    if (!setup_walk(ctx, order, entity, np, bins, g)) return PIC_ERR_UNSUPPORTED;
+    if (const int rc = setup_walk(ctx, order, entity, np, bins, g)) return rc;
     g.a = make_poly_args<T>(ctx, entity, ncomp, mesh, np, x, nullptr, nullptr, c, gu, nullptr,
                             0.0, bins.perm);
     const bool stag = stag_dim(entity) >= 0;
