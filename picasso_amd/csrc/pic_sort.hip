@@ -5,11 +5,17 @@
 // then by input index.
 //
 // Passes
-//  (1) keys in input order + per-brick counts (wave-aggregated atomics);
+//  (1) keys in input order, written to the caller's keys array, per-brick
+//      counts (wave-aggregated atomics), and a flag raised when some key is
+//      smaller than its predecessor's;
 //  (2) exclusive scan of the counts into bin_start, and the list of
 //      non-empty bricks, in two passes over tiles of counts;
-//  (3) scatter of particle indices into their brick segments (wave-aggregated
-//      cursors: order inside a segment depends on scheduling);
+//  (3) flag down (keys already non-decreasing in input order, as when the
+//      particles are stored in bin order): the stable sort is the identity,
+//      perm[s] = s and the keys are already in place; passes 3-4 do nothing
+//      more.  Flag up: scatter of particle indices and their keys into their
+//      brick segments (wave-aggregated cursors: order inside a segment
+//      depends on scheduling);
 //  (4) one workgroup per brick: LDS counting sort by bin, then each bin's
 //      indices sorted ascending, which restores the stable order bit-exactly
 //      whatever order pass 3 produced.  Writes perm and the sorted keys.
@@ -96,13 +102,20 @@ constexpr int SORT_ITEMS = 4;
 constexpr int SORT_PT = 256;  // work-items per block of those passes
 
 // Particles outside the bins are counted in count[nbricks] (they are placed
-// after every brick, so perm stays a permutation even then).
+// after every brick, so perm stays a permutation even then).  Keys go to
+// keys[] in input order.  *unsorted is raised (by one atomic per
+// workgroup) when some particle's key is smaller than its predecessor's;
+// the predecessor of a workgroup's first particle is recomputed from its
+// position.
 template <typename T>
 __global__ __launch_bounds__(SORT_PT) void k_cell_keys(int64_t np, PField<T> x, KeyGeom g,
                                                        uint32_t nbricks, uint32_t* keys,
-                                                       uint32_t* count, int* err)
+                                                       uint32_t* count, uint32_t* unsorted,
+                                                       int* err)
 {
-    const int64_t p0 = int64_t(blockIdx.x) * (SORT_PT * SORT_ITEMS) + threadIdx.x;
+    __shared__ uint32_t skey[SORT_PT * SORT_ITEMS];
+    const int64_t blk0 = int64_t(blockIdx.x) * (SORT_PT * SORT_ITEMS);
+    const int64_t p0 = blk0 + threadIdx.x;
     uint32_t key[SORT_ITEMS], b[SORT_ITEMS];
 #pragma unroll
     for (int k = 0; k < SORT_ITEMS; ++k) {
@@ -111,10 +124,16 @@ __global__ __launch_bounds__(SORT_PT) void k_cell_keys(int64_t np, PField<T> x, 
         b[k] = 0;
         if (p < np) key[k] = cell_key(g, x, p, &b[k]);
     }
+    uint32_t before = 0u;  // key of the workgroup's predecessor particle
+    if (threadIdx.x == 0 && blk0 > 0) {
+        uint32_t bb;
+        before = cell_key(g, x, blk0 - 1, &bb);
+    }
 #pragma unroll
     for (int k = 0; k < SORT_ITEMS; ++k) {
         const int64_t p = p0 + k * SORT_PT;
         const bool live = p < np;
+        skey[k * SORT_PT + threadIdx.x] = key[k];
         if (live) {
             keys[p] = key[k];
             if (key[k] == 0xffffffffu) {
@@ -124,6 +143,16 @@ __global__ __launch_bounds__(SORT_PT) void k_cell_keys(int64_t np, PField<T> x, 
         }
         (void)wave_agg_inc(count, b[k], live);
     }
+    __syncthreads();
+    bool descent = false;
+#pragma unroll
+    for (int k = 0; k < SORT_ITEMS; ++k) {
+        const int q = k * SORT_PT + int(threadIdx.x);
+        if (blk0 + q >= np) continue;
+        const uint32_t prev = q > 0 ? skey[q - 1] : before;
+        descent = descent || key[k] < prev;
+    }
+    if (__syncthreads_or(descent) && threadIdx.x == 0) atomicOr(unsorted, 1u);
 }
 
 // ---------------------------------------------------------------------------
@@ -241,15 +270,26 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_compact(const uint32_t* c
 
 
 // ---------------------------------------------------------------------------
-// Particles outside the bins go straight to their final slots after the
-// last brick (key 0xffffffff, order unspecified: the call reports an error).
+// Keys already non-decreasing (flag down): the stable sort is the identity.
+// Otherwise every particle index goes to its brick segment of tmp, its key
+// to the same slot of tmpk; the outsiders (key 0xffffffff) fill the segment
+// after the last brick, in an order fixed by the brick pass.
 __global__ __launch_bounds__(SORT_PT) void k_bin_scatter(int64_t np, const uint32_t* keys,
                                                          int32_t vshift, uint32_t bvol,
                                                          uint32_t nbricks, const uint32_t* start,
+                                                         const uint32_t* unsorted,
                                                          uint32_t* cursor, uint32_t* tmp,
-                                                         uint32_t* perm, uint32_t* keys_out)
+                                                         uint32_t* tmpk, uint32_t* perm)
 {
     const int64_t p0 = int64_t(blockIdx.x) * (SORT_PT * SORT_ITEMS) + threadIdx.x;
+    if (*unsorted == 0u) {
+#pragma unroll
+        for (int k = 0; k < SORT_ITEMS; ++k) {
+            const int64_t p = p0 + k * SORT_PT;
+            if (p < np) perm[p] = uint32_t(p);
+        }
+        return;
+    }
     uint32_t key[SORT_ITEMS];
 #pragma unroll
     for (int k = 0; k < SORT_ITEMS; ++k) {
@@ -272,28 +312,24 @@ __global__ __launch_bounds__(SORT_PT) void k_bin_scatter(int64_t np, const uint3
         const int64_t p = p0 + k * SORT_PT;
         const uint32_t r = wave_agg_resolve(tk[k]);
         if (p >= np) continue;
-        if (key[k] != 0xffffffffu) {
-            tmp[at[k] + r] = uint32_t(p);
-        } else {
-            perm[at[k] + r] = uint32_t(p);
-            keys_out[at[k] + r] = 0xffffffffu;
-        }
+        tmp[at[k] + r] = uint32_t(p);
+        tmpk[at[k] + r] = key[k];
     }
 }
 
 // Sort one brick segment held in `items` (LDS or global scratch): count the
 // bins, scan, place each index in its bin (unordered), then give each index
-// its rank among the indices of its bin, which is its stable position.
+// its rank among the indices of its bin, which is its stable position.  The
+// segment's indices are tmp[beg, beg + n), their keys tmpk[beg, beg + n).
 __device__ void brick_segment_sort(uint32_t b, uint32_t beg, uint32_t n, uint32_t bvol,
-                                   const uint32_t* tmp, const uint32_t* keys_in,
+                                   const uint32_t* tmp, const uint32_t* tmpk,
                                    uint32_t* items, uint32_t* cnt, uint32_t* cur,
                                    uint32_t* perm, uint32_t* keys_out)
 {
     const uint32_t kb = b * bvol;
     for (uint32_t l = threadIdx.x; l < bvol; l += blockDim.x) cnt[l] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
-        atomicAdd(&cnt[keys_in[tmp[beg + i]] - kb], 1u);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) atomicAdd(&cnt[tmpk[beg + i] - kb], 1u);
     __syncthreads();
     // exclusive scan of cnt into cur (one wave, 64 bins at a time)
     if (threadIdx.x < 64) {
@@ -313,16 +349,22 @@ __device__ void brick_segment_sort(uint32_t b, uint32_t beg, uint32_t n, uint32_
     }
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const uint32_t p = tmp[beg + i];
-        const uint32_t pos = atomicAdd(&cur[keys_in[p] - kb], 1u);
-        items[pos] = p;
+        const uint32_t pos = atomicAdd(&cur[tmpk[beg + i] - kb], 1u);
+        items[pos] = tmp[beg + i];
     }
     __syncthreads();
-    // cur[l] is now the end of bin l.  Indices are distinct, so the ranks
-    // inside a bin are a permutation of its slots.
+    // cur[l] is now the end of bin l; item i lies in the first bin whose end
+    // exceeds i.  Indices are distinct, so the ranks inside a bin are a
+    // permutation of its slots.
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
         const uint32_t v = items[i];
-        const uint32_t l = keys_in[v] - kb;
+        uint32_t lo = 0, hi = bvol - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (cur[mid] > i) hi = mid;
+            else lo = mid + 1;
+        }
+        const uint32_t l = lo;
         const uint32_t e = cur[l];
         uint32_t r = e - cnt[l];
         for (uint32_t j = e - cnt[l]; j < e; ++j) r += items[j] < v ? 1u : 0u;
@@ -332,7 +374,7 @@ __device__ void brick_segment_sort(uint32_t b, uint32_t beg, uint32_t n, uint32_
 }
 
 // brick_segment_sort for segments of at most SORT_REG * blockDim.x
-// indices: each index and its bin are gathered from memory once, into
+// indices: each index and its key are loaded once (coalesced) into
 // registers; the bin count's LDS atomic returns the index's arrival rank,
 // which places it in its bin with no second pass over memory, and the bin
 // of every placed index is kept in LDS (ibin) for the rank step.
@@ -340,7 +382,7 @@ constexpr int SORT_REG = 4;
 static_assert(SORT_REG * SORT_BT <= SORT_LDS_CAP, "register-path segments fit the LDS arrays");
 
 __device__ void brick_segment_sort_reg(uint32_t b, uint32_t beg, uint32_t n, uint32_t bvol,
-                                       const uint32_t* tmp, const uint32_t* keys_in,
+                                       const uint32_t* tmp, const uint32_t* tmpk,
                                        uint32_t* items, uint32_t* ibin, uint32_t* cnt,
                                        uint32_t* cur, uint32_t* perm, uint32_t* keys_out)
 {
@@ -350,13 +392,9 @@ __device__ void brick_segment_sort_reg(uint32_t b, uint32_t beg, uint32_t n, uin
     for (int r = 0; r < SORT_REG; ++r) {
         const uint32_t i = threadIdx.x + uint32_t(r) * blockDim.x;
         pv[r] = i < n ? tmp[beg + i] : 0u;
+        lb[r] = i < n ? tmpk[beg + i] - kb : 0u;
     }
     for (uint32_t l = threadIdx.x; l < bvol; l += blockDim.x) cnt[l] = 0;
-#pragma unroll
-    for (int r = 0; r < SORT_REG; ++r) {
-        const uint32_t i = threadIdx.x + uint32_t(r) * blockDim.x;
-        lb[r] = i < n ? keys_in[pv[r]] - kb : 0u;
-    }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < SORT_REG; ++r) {
@@ -404,31 +442,41 @@ __device__ void brick_segment_sort_reg(uint32_t b, uint32_t beg, uint32_t n, uin
     }
 }
 
-// Workgroups walk the non-empty bricks with a grid stride.  Dynamic LDS:
-// cnt[bvol] | cur[bvol] | items[SORT_LDS_CAP] | ibin[SORT_LDS_CAP]; bricks
-// of more than SORT_LDS_CAP indices use `big`.
-__global__ __launch_bounds__(SORT_BT) void k_brick_sort(const uint32_t* start,
+// Workgroups walk the non-empty bricks with a grid stride; the segment of
+// particles outside the bins, [start[nbricks], np), is copied as it was
+// scattered (key 0xffffffff).  Nothing to do when the keys arrived sorted.
+// Dynamic LDS: cnt[bvol] | cur[bvol] | items[SORT_LDS_CAP] |
+// ibin[SORT_LDS_CAP]; bricks of more than SORT_LDS_CAP indices use `big`.
+__global__ __launch_bounds__(SORT_BT) void k_brick_sort(int64_t np, uint32_t nbricks,
+                                                        const uint32_t* start,
                                                         const uint32_t* active,
+                                                        const uint32_t* unsorted,
                                                         const uint32_t* tmp,
-                                                        const uint32_t* keys_in, uint32_t bvol,
+                                                        const uint32_t* tmpk, uint32_t bvol,
                                                         uint32_t* big, uint32_t* perm,
                                                         uint32_t* keys_out)
 {
+    if (*unsorted == 0u) return;
     extern __shared__ uint32_t sort_lds[];
     uint32_t* cnt = sort_lds;
     uint32_t* cur = sort_lds + bvol;
     uint32_t* lds_items = sort_lds + 2 * bvol;
     uint32_t* ibin = lds_items + SORT_LDS_CAP;
+    for (int64_t s = int64_t(start[nbricks]) + int64_t(blockIdx.x) * SORT_BT + threadIdx.x;
+         s < np; s += int64_t(gridDim.x) * SORT_BT) {
+        perm[s] = tmp[s];
+        keys_out[s] = 0xffffffffu;
+    }
     const uint32_t nact = active[0];
     for (uint32_t t = blockIdx.x; t < nact; t += gridDim.x) {
         const uint32_t b = active[1 + t];
         const uint32_t beg = start[b], n = start[b + 1] - beg;
         if (n <= uint32_t(SORT_REG * SORT_BT)) {
-            brick_segment_sort_reg(b, beg, n, bvol, tmp, keys_in, lds_items, ibin, cnt, cur, perm,
+            brick_segment_sort_reg(b, beg, n, bvol, tmp, tmpk, lds_items, ibin, cnt, cur, perm,
                                    keys_out);
         } else {
             uint32_t* items = n <= SORT_LDS_CAP ? lds_items : big + beg;
-            brick_segment_sort(b, beg, n, bvol, tmp, keys_in, items, cnt, cur, perm, keys_out);
+            brick_segment_sort(b, beg, n, bvol, tmp, tmpk, items, cnt, cur, perm, keys_out);
         }
         __syncthreads();
     }
@@ -497,36 +545,36 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
     DeviceGuard dg(ctx->device);
     if (!dg.ok) return PIC_ERR_HIP;
 
-    // scratch: count[nb+1] | cursor[nb+1] | tile sums (64-bit) | act[nb+1] |
-    // keys_in[np] | tmp[np] | big[np].  act holds the active-brick list when
-    // the caller does not ask for it.
+    // scratch: count[nb+1] | cursor[nb+1] | unsorted flag (2 words) | tile
+    // sums (64-bit) | act[nb+1] | tmp[np] | tmpk[np] | big[np].  act holds
+    // the active-brick list when the caller does not ask for it.
     const int64_t ntile = (nb + 1 + SCAN_TILE - 1) / SCAN_TILE;
-    const int64_t head = (nb + 1) + (nb + 1) + 2 * ntile + (active ? 0 : nb + 1);
+    const int64_t nzero = 2 * (nb + 1) + 2;  // counts, cursors and flag: cleared first
+    const int64_t head = nzero + 2 * ntile + (active ? 0 : nb + 1);
     const size_t bytes = size_t(head + 3 * np) * 4;
     uint32_t* sc = static_cast<uint32_t*>(scratch(ctx, bytes));
     if (!sc) return PIC_ERR_ALLOC;
     uint32_t* count = sc;
     uint32_t* cursor = count + nb + 1;
-    uint64_t* tile_sums = reinterpret_cast<uint64_t*>(cursor + nb + 1);
+    uint32_t* unsorted = cursor + nb + 1;
+    uint64_t* tile_sums = reinterpret_cast<uint64_t*>(sc + nzero);
     uint32_t* act = active ? active : reinterpret_cast<uint32_t*>(tile_sums + ntile);
-    uint32_t* keys_in = sc + head;
-    uint32_t* tmp = keys_in + np;
-    uint32_t* big = tmp + np;
+    uint32_t* tmp = sc + head;
+    uint32_t* tmpk = tmp + np;
+    uint32_t* big = tmpk + np;
     hipStream_t st = ctx->stream;
 
     TimedLaunch tl(ctx, "bin_sort");
-    const unsigned fb = (unsigned)std::min<int64_t>((2 * nb + 2 + 255) / 256, 4096);
-    k_fill_u32<<<fb, 256, 0, st>>>(count, 2 * nb + 2, 0u);
+    const unsigned fb = (unsigned)std::min<int64_t>((nzero + 255) / 256, 4096);
+    k_fill_u32<<<fb, 256, 0, st>>>(count, nzero, 0u);
     if (np > 0) {
         const unsigned b = blocks_for(np, SORT_PT * SORT_ITEMS);
         if (dtype == PIC_F64)
             k_cell_keys<double><<<b, SORT_PT, 0, st>>>(np, make_pfield<double>(*x), g, uint32_t(nb),
-                                                    keys_in, count,
-                                                    ctx->d_err);
+                                                    keys, count, unsorted, ctx->d_err);
         else
             k_cell_keys<float><<<b, SORT_PT, 0, st>>>(np, make_pfield<float>(*x), g, uint32_t(nb),
-                                                   keys_in, count,
-                                                   ctx->d_err);
+                                                   keys, count, unsorted, ctx->d_err);
     }
     k_tile_sums<<<(unsigned)ntile, SCAN_THREADS, 0, st>>>(count, nb, tile_sums);
     const bool prefixed = ntile > SCAN_TILE;
@@ -535,11 +583,12 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
                                                              bin_start, act);
     if (np > 0) {
         k_bin_scatter<<<blocks_for(np, SORT_PT * SORT_ITEMS), SORT_PT, 0, st>>>(
-            np, keys_in, g.vshift, g.bvol, uint32_t(nb), bin_start, cursor, tmp, perm, keys);
+            np, keys, g.vshift, g.bvol, uint32_t(nb), bin_start, unsorted, cursor, tmp, tmpk,
+            perm);
         const size_t lds = size_t(2 * g.bvol + 2 * SORT_LDS_CAP) * 4;
-        const unsigned nbk = persistent_blocks(ctx, k_brick_sort, SORT_BT, lds, nb);
-        k_brick_sort<<<nbk, SORT_BT, lds, st>>>(bin_start, act, tmp, keys_in, g.bvol, big, perm,
-                                                keys);
+        const unsigned nbk = persistent_blocks(ctx, k_brick_sort, SORT_BT, lds, nb + 1);
+        k_brick_sort<<<nbk, SORT_BT, lds, st>>>(np, uint32_t(nb), bin_start, act, unsorted, tmp,
+                                                tmpk, g.bvol, big, perm, keys);
     }
     return launch_status();
 }
