@@ -38,6 +38,7 @@ struct alignas(16) BrickItem {
 // of the list.
 constexpr int WALK_LANES = 8;
 constexpr int WALK_PAD = 64;  // words between counters (256 B)
+constexpr uint32_t NUM_XCD = 8;  // accelerator complex dies of an MI355X, one L2 each
 
 template <int BB, typename G>
 __device__ __forceinline__ void brick_origin(const G& g, uint32_t b, int f0[3])
@@ -147,6 +148,12 @@ __device__ __forceinline__ void walk_bricks(const G& g, BrickItem* q, Body&& bod
         nl = WALK_LANES;
         lane = (blockIdx.x + m) % WALK_LANES;
         t0 = lane + WALK_LANES * m;
+    } else if (!DYN && gridDim.x % NUM_XCD == 0) {
+        // Static walk: workgroups are dealt to the XCDs round-robin by
+        // blockIdx, so give each XCD a contiguous run of gridDim / 8 walk
+        // indices per round; neighbouring bricks then share their halo
+        // rows in one XCD's L2 instead of fetching them into eight.
+        t0 = (blockIdx.x % NUM_XCD) * (gridDim.x / NUM_XCD) + blockIdx.x / NUM_XCD;
     }
     BrickWalk<DYN, G> w{g, q, walk_count(g), lane, nl, 0, 0u, BrickItem{}};
     if (threadIdx.x == 0) q[0] = load_item(g, t0, w.nw);

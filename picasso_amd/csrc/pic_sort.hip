@@ -39,30 +39,26 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
         p[i] = v;
 }
 
-// Wave-aggregated atomicAdd of 1 per active lane on base[key]: lanes with
-// equal keys share one atomic.  Returns each lane's slot (old value + rank of
-// the lane among equal-key lanes, in lane order).
-__device__ __forceinline__ uint32_t wave_agg_inc(uint32_t* base, uint32_t key, bool active)
+// Wave-aggregated count: atomicAdd of 1 per active lane on base[key], lanes
+// with equal keys sharing one atomic.  Nothing is returned, so the atomics
+// are fire-and-forget and a work-item's later items do not wait for them.
+__device__ __forceinline__ void wave_agg_count(uint32_t* base, uint32_t key, bool active)
 {
     const int lane = __lane_id();
     uint64_t pending = __ballot(active);
-    uint32_t slot = 0;
     while (pending) {
         const int leader = __ffsll((unsigned long long)pending) - 1;
         const uint32_t lkey = __shfl(key, leader);
         const uint64_t same = __ballot(active && key == lkey) & pending;
-        uint32_t b = 0;
-        if (lane == leader) b = atomicAdd(base + lkey, (uint32_t)__popcll(same));
-        b = __shfl(b, leader);
-        if ((same >> lane) & 1ull) slot = b + (uint32_t)__popcll(same & ((1ull << lane) - 1ull));
+        if (lane == leader) (void)atomicAdd(base + lkey, (uint32_t)__popcll(same));
         pending &= ~same;
     }
-    return slot;
 }
 
-// Two-phase form of wave_agg_inc for several calls in flight: issue() does
-// the aggregated atomics and keeps, per lane, its group's leader and its
-// rank; resolve() reads the leader's returned base.  The atomics of all
+// Wave-aggregated slot claim in two phases, for several calls in flight:
+// issue() does the aggregated atomics and keeps, per lane, its group's
+// leader and its rank (among equal-key lanes, in lane order); resolve()
+// reads the leader's returned base.  The atomics of all
 // issued calls are then outstanding together instead of one at a time.
 struct AggTicket {
     uint32_t base;  // returned old value (meaningful in the leader lane)
@@ -141,7 +137,7 @@ __global__ __launch_bounds__(SORT_PT) void k_cell_keys(int64_t np, PField<T> x, 
                 b[k] = nbricks;
             }
         }
-        (void)wave_agg_inc(count, b[k], live);
+        wave_agg_count(count, b[k], live);
     }
     __syncthreads();
     bool descent = false;
@@ -531,42 +527,60 @@ KeyGeom make_key_geom(const pic_mesh& mesh, const pic_binning& b)
 
 }  // namespace pic
 
-extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, int64_t np,
-                            const pic_pfield* x, const pic_binning* geom, uint32_t* keys,
-                            uint32_t* perm, uint32_t* bin_start, uint32_t* active)
+namespace pic {
+
+// Scratch of one sort: count[nb+1] | cursor[nb+1] | unsorted flag (2 words)
+// | tile sums (64-bit) | act[nb+1] | tmp[np] | tmpk[np] | big[np], in
+// 4-byte words.  act holds the active-brick list when the caller does not
+// ask for it.  The first nzero words must start at zero (the first pass
+// clears them).
+struct SortLayout {
+    int64_t ntile, nzero, head;
+    size_t bytes;
+};
+
+static SortLayout sort_layout(int64_t nb, int64_t np, bool caller_active)
+{
+    SortLayout l;
+    l.ntile = (nb + 1 + SCAN_TILE - 1) / SCAN_TILE;
+    l.nzero = 2 * (nb + 1) + 2;
+    l.head = l.nzero + 2 * l.ntile + (caller_active ? 0 : nb + 1);
+    l.bytes = size_t(l.head + 3 * np) * 4;
+    return l;
+}
+
+static int check_sort_args(pic_context* ctx, int dtype, const pic_mesh* mesh, int64_t np,
+                           const pic_pfield* x, const pic_binning* geom, const uint32_t* keys,
+                           const uint32_t* perm, const uint32_t* bin_start, int64_t* nb)
 {
     if (!ctx || !mesh || !(mesh->h > 0.0) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
         np >= int64_t(UINT32_MAX) || !geom || !bin_start)
         return PIC_ERR_ARG;
     if (np > 0 && (!valid_pfield(x) || !keys || !perm)) return PIC_ERR_ARG;
-    const int64_t nb = pic_bin_count(geom);
-    if (nb < 0) return (int)nb;
-    const KeyGeom g = make_key_geom(*mesh, *geom);
-    DeviceGuard dg(ctx->device);
-    if (!dg.ok) return PIC_ERR_HIP;
+    *nb = pic_bin_count(geom);
+    return *nb < 0 ? int(*nb) : PIC_OK;
+}
 
-    // scratch: count[nb+1] | cursor[nb+1] | unsorted flag (2 words) | tile
-    // sums (64-bit) | act[nb+1] | tmp[np] | tmpk[np] | big[np].  act holds
-    // the active-brick list when the caller does not ask for it.
-    const int64_t ntile = (nb + 1 + SCAN_TILE - 1) / SCAN_TILE;
-    const int64_t nzero = 2 * (nb + 1) + 2;  // counts, cursors and flag: cleared first
-    const int64_t head = nzero + 2 * ntile + (active ? 0 : nb + 1);
-    const size_t bytes = size_t(head + 3 * np) * 4;
-    uint32_t* sc = static_cast<uint32_t*>(scratch(ctx, bytes));
-    if (!sc) return PIC_ERR_ALLOC;
+// The sort's passes on the context's stream with scratch `sc` laid out by
+// sort_layout (device memory of at least L.bytes).
+static int bin_sort_run(pic_context* ctx, int dtype, const pic_mesh& mesh, int64_t np,
+                        const pic_pfield* x, const pic_binning& geom, int64_t nb,
+                        const SortLayout& L, uint32_t* sc, uint32_t* keys, uint32_t* perm,
+                        uint32_t* bin_start, uint32_t* active)
+{
+    const KeyGeom g = make_key_geom(mesh, geom);
     uint32_t* count = sc;
     uint32_t* cursor = count + nb + 1;
     uint32_t* unsorted = cursor + nb + 1;
-    uint64_t* tile_sums = reinterpret_cast<uint64_t*>(sc + nzero);
-    uint32_t* act = active ? active : reinterpret_cast<uint32_t*>(tile_sums + ntile);
-    uint32_t* tmp = sc + head;
+    uint64_t* tile_sums = reinterpret_cast<uint64_t*>(sc + L.nzero);
+    uint32_t* act = active ? active : reinterpret_cast<uint32_t*>(tile_sums + L.ntile);
+    uint32_t* tmp = sc + L.head;
     uint32_t* tmpk = tmp + np;
     uint32_t* big = tmpk + np;
     hipStream_t st = ctx->stream;
 
-    TimedLaunch tl(ctx, "bin_sort");
-    const unsigned fb = (unsigned)std::min<int64_t>((nzero + 255) / 256, 4096);
-    k_fill_u32<<<fb, 256, 0, st>>>(count, nzero, 0u);
+    const unsigned fb = (unsigned)std::min<int64_t>((L.nzero + 255) / 256, 4096);
+    k_fill_u32<<<fb, 256, 0, st>>>(count, L.nzero, 0u);
     if (np > 0) {
         const unsigned b = blocks_for(np, SORT_PT * SORT_ITEMS);
         if (dtype == PIC_F64)
@@ -576,11 +590,11 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
             k_cell_keys<float><<<b, SORT_PT, 0, st>>>(np, make_pfield<float>(*x), g, uint32_t(nb),
                                                    keys, count, unsorted, ctx->d_err);
     }
-    k_tile_sums<<<(unsigned)ntile, SCAN_THREADS, 0, st>>>(count, nb, tile_sums);
-    const bool prefixed = ntile > SCAN_TILE;
-    if (prefixed) k_scan_sums<<<1, SCAN_THREADS, 0, st>>>(tile_sums, ntile);
-    k_scan_compact<<<(unsigned)ntile, SCAN_THREADS, 0, st>>>(count, nb, tile_sums, prefixed,
-                                                             bin_start, act);
+    k_tile_sums<<<(unsigned)L.ntile, SCAN_THREADS, 0, st>>>(count, nb, tile_sums);
+    const bool prefixed = L.ntile > SCAN_TILE;
+    if (prefixed) k_scan_sums<<<1, SCAN_THREADS, 0, st>>>(tile_sums, L.ntile);
+    k_scan_compact<<<(unsigned)L.ntile, SCAN_THREADS, 0, st>>>(count, nb, tile_sums, prefixed,
+                                                               bin_start, act);
     if (np > 0) {
         k_bin_scatter<<<blocks_for(np, SORT_PT * SORT_ITEMS), SORT_PT, 0, st>>>(
             np, keys, g.vshift, g.bvol, uint32_t(nb), bin_start, unsorted, cursor, tmp, tmpk,
@@ -591,6 +605,24 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
                                                 tmpk, g.bvol, big, perm, keys);
     }
     return launch_status();
+}
+
+}  // namespace pic
+
+extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, int64_t np,
+                            const pic_pfield* x, const pic_binning* geom, uint32_t* keys,
+                            uint32_t* perm, uint32_t* bin_start, uint32_t* active)
+{
+    int64_t nb = 0;
+    if (const int rc = check_sort_args(ctx, dtype, mesh, np, x, geom, keys, perm, bin_start, &nb))
+        return rc;
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok) return PIC_ERR_HIP;
+    const SortLayout L = sort_layout(nb, np, active != nullptr);
+    uint32_t* sc = static_cast<uint32_t*>(pic::scratch(ctx, L.bytes));
+    if (!sc) return PIC_ERR_ALLOC;
+    TimedLaunch tl(ctx, "bin_sort");
+    return bin_sort_run(ctx, dtype, *mesh, np, x, *geom, nb, L, sc, keys, perm, bin_start, active);
 }
 
 // ---------------------------------------------------------------------------
