@@ -310,6 +310,24 @@ int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh,
                  uint32_t* keys, uint32_t* perm, uint32_t* bin_start,
                  uint32_t* active);
 
+/* pic_bin_sort without any device allocation (no reference counterpart:
+ * for callers that cannot allocate -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * and errors as pic_bin_sort, with the sort's scratch in one array of
+ * PIC_NOALLOC_SORT_BYTES that the library keeps on every GPU it runs on until
+ * the process ends, instead of the context's growable scratch.  Returns
+ * PIC_ERR_CAPACITY when the sort needs more than that: the scratch takes
+ * 4 * (3 np + 2 (nbricks + 1) + 2 + 2 ceil((nbricks + 1) / 2048)) bytes, plus
+ * 4 (nbricks + 1) when active is NULL; np <= 4,194,304 with nbricks <
+ * 1,048,576 always fits.  Every call on one GPU shares that GPU's array
+ * (whatever the context): calls that overlap in time on one GPU race. */
+#define PIC_NOALLOC_SORT_BYTES (64u << 20)
+int pic_bin_sort_noalloc(pic_context* ctx, int dtype, const pic_mesh* mesh,
+                         int64_t np, const pic_pfield* x,
+                         const pic_binning* geom, uint32_t* keys,
+                         uint32_t* perm, uint32_t* bin_start,
+                         uint32_t* active);
+
 /* Gather particle fields into bin order (no reference function; Picasso
  * never reorders its particles): for each of nfields fields, dst(s, c) =
  * src(perm[s], c) for s < np and c < ncomp[field].  src and dst must not

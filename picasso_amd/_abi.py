@@ -86,6 +86,8 @@ PROTOTYPES = {
     "pic_bin_count": (_c_i64, [_P(Binning)]),
     "pic_bin_sort": (_c_int, [_vp, _c_int, _P(Mesh), _c_i64, _P(PField), _P(Binning), _vp,
                               _vp, _vp, _vp]),
+    "pic_bin_sort_noalloc": (_c_int, [_vp, _c_int, _P(Mesh), _c_i64, _P(PField), _P(Binning),
+                                      _vp, _vp, _vp, _vp]),
     "pic_apic_g2p_normalized": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(Mesh), _c_i64,
                                          _P(PField), _P(Grid), _P(Grid), _P(PField), _P(Bins)]),
     "pic_apic_g2p_update_F": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(Mesh), _c_i64,

@@ -170,7 +170,7 @@ int pic_context_create(pic_context** out, int device)
         return PIC_ERR_ALLOC;
     }
     if (hipMemset(ctx->d_err, 0, sizeof(int)) != hipSuccess ||
-        hipDeviceSynchronize() != hipSuccess) {
+        hipDeviceSynchronize() != hipSuccess || noalloc_setup(device) != PIC_OK) {
         (void)hipFree(ctx->d_err);
         delete ctx;
         return PIC_ERR_HIP;

@@ -126,6 +126,10 @@ unsigned persistent_blocks(pic_context* ctx, K kernel, int threads, size_t dyn_l
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>(g, items));
 }
 
+// Resolve, once per device and with that device current, the device address
+// of the allocation-free sort's scratch array (pic_bin_sort_noalloc).
+int noalloc_setup(int device);
+
 // Zero a grid window on the stream (used by PIC_P2G_ZERO_FIRST).
 int zero_grid(pic_context* ctx, int dtype, const pic_grid& g, int ncomp);
 

@@ -19,6 +19,8 @@
 //  (4) one workgroup per brick: LDS counting sort by bin, then each bin's
 //      indices sorted ascending, which restores the stable order bit-exactly
 //      whatever order pass 3 produced.  Writes perm and the sorted keys.
+#include <mutex>
+
 #include "pic_common.hpp"
 #include "pic_internal.hpp"
 #include "pic_sort.hpp"
@@ -706,4 +708,57 @@ extern "C" int pic_select(pic_context* ctx, int64_t np, const uint8_t* keep, uin
     k_scan_compact<<<(unsigned)ntile, SCAN_THREADS, 0, st>>>(count, np, tile_sums, prefixed, scan,
                                                              kept);
     return launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free bin sort: the same passes with their scratch in a
+// file-scope device array of PIC_NOALLOC_SORT_BYTES, which every GPU the
+// library runs on holds until the process ends.  In host code the array's
+// name is not a device address: the address is resolved per device with
+// hipGetSymbolAddress and only that pointer is handed to kernels.
+static __device__ __align__(256) uint32_t scratch[PIC_NOALLOC_SORT_BYTES / 4];
+
+namespace {
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex noalloc_mu;
+void* noalloc_ptr[NOALLOC_MAX_DEVICES];  // device address of `scratch`, per device
+}  // namespace
+
+namespace pic {
+
+int noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return PIC_ERR_ARG;
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (noalloc_ptr[device]) return PIC_OK;
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess || cur != device) return PIC_ERR_HIP;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)) != hipSuccess || !p) return PIC_ERR_HIP;
+    noalloc_ptr[device] = p;
+    return PIC_OK;
+}
+
+}  // namespace pic
+
+extern "C" int pic_bin_sort_noalloc(pic_context* ctx, int dtype, const pic_mesh* mesh, int64_t np,
+                                    const pic_pfield* x, const pic_binning* geom, uint32_t* keys,
+                                    uint32_t* perm, uint32_t* bin_start, uint32_t* active)
+{
+    int64_t nb = 0;
+    if (const int rc = check_sort_args(ctx, dtype, mesh, np, x, geom, keys, perm, bin_start, &nb))
+        return rc;
+    const SortLayout L = sort_layout(nb, np, active != nullptr);
+    if (L.bytes > sizeof(::scratch)) return PIC_ERR_CAPACITY;
+    if (ctx->device < 0 || ctx->device >= NOALLOC_MAX_DEVICES) return PIC_ERR_ARG;
+    DeviceGuard dg(ctx->device);
+    if (!dg.ok) return PIC_ERR_HIP;
+    if (const int rc = noalloc_setup(ctx->device)) return rc;
+    void* p = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(noalloc_mu);
+        p = noalloc_ptr[ctx->device];
+    }
+    return bin_sort_run(ctx, dtype, *mesh, np, x, *geom, nb, L, static_cast<uint32_t*>(p), keys,
+                        perm, bin_start, active);
 }
