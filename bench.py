@@ -25,6 +25,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The CPU baseline's OpenMP threads stay on their cores; the runtime reads
+# these when it starts, before torch is imported.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 METRIC = ("APIC p2g+g2p particles/s @256³×8ppc, FP32-comp vs FP64 at stated err; "
           "HBM GB/s")
@@ -53,6 +57,7 @@ class Step:
         self.c = torch.as_tensor(np.ascontiguousarray(c.reshape(P, 12).T), dtype=dtype,
                                  device=dev).t()
         self.c0 = self.c.clone()
+        self.order = np.arange(P)  # input index of the particle in each storage slot
         lo, n = wl.node_window
         nn = tuple(int(v) for v in n)
         self.gm = pa.GridField(torch.zeros(nn, dtype=dtype, device=dev))
@@ -86,6 +91,14 @@ class Step:
         for k, v in new.items():
             setattr(self, k, v)
         self.c0 = self.c.clone()
+        self.order = self.order[self.bins.perm.long().cpu().numpy()]
+
+    def outputs(self):
+        """Grid mass, momentum and c_p in input order, as float64 numpy."""
+        c = np.empty((len(self.order), 12))
+        c[self.order] = self.c.double().cpu().numpy()
+        return (self.gm.tensor.double().cpu().numpy(),
+                self.gmu.tensor.double().cpu().numpy(), c.reshape(-1, 4, 3))
 
     def run(self):
         pa, ctx = self.pa, self.ctx
@@ -150,24 +163,102 @@ def time_graph_steps(step, steps, warmup, ctx):
     return (t1 - t0) * 1e3 / steps
 
 
-def cpu_baseline(wl, reps=3):
-    """Oracle OpenMP FP64 step (atomic p2g) on the host cores."""
+def _cpu_info():
+    """CPU model and physical core count of this host, and the CPUs this
+    process may run on."""
+    model, cores = None, set()
+    try:
+        phys = core = None
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model is None:
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count()
+    return model, len(cores) or None, avail
+
+
+def oracle_step(wl, x, m, c, threads=0):
+    """The oracle's FP64 p2g -> normalise -> g2p on a zeroed node window:
+    grid mass, momentum (window shaped) and c_p."""
     from oracle import orpic
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    lo, n = wl.node_window
+    cc = np.ascontiguousarray(c.copy())
+    gm, gmu, _ = orpic.apic_step_omp(2, wl.low, wl.h, lo, n, np.ascontiguousarray(x), m, cc,
+                                     mode=1, nthreads=threads)
+    shp = tuple(int(v) for v in n)
+    return gm.reshape(shp), gmu.reshape(shp + (3,)), cc.reshape(-1, 4, 3)
+
+
+def field_errors(got, ref):
+    """||got - ref||_inf / ||ref||_inf of grid mass, grid momentum, particle
+    velocity (c_p row 0) and B (rows 1-3)."""
+    def rel(a, b):
+        return float(np.abs(a - b).max() / np.abs(b).max())
+    return {"grid_mass": rel(got[0], ref[0]), "grid_momentum": rel(got[1], ref[1]),
+            "particle_velocity": rel(got[2][:, 0], ref[2][:, 0]),
+            "particle_B": rel(got[2][:, 1:], ref[2][:, 1:])}
+
+
+def cpu_baseline(wl, synth, reps=5):
+    """The oracle's OpenMP FP64 step (atomic p2g) on the host cores of the
+    process: the whole workload, median of `reps` runs after one warm-up,
+    on every thread OpenMP may use here (OMP_NUM_THREADS, else the CPUs
+    this process may run on), threads bound to cores; and a one-thread run
+    on the 32^3 x 8 ppc workload.  Returns the baseline record and the
+    oracle's outputs of the full workload."""
+    from oracle import orpic
+    model, phys, avail = _cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
     lo, n = wl.node_window
     x = np.ascontiguousarray(wl.x)
-    c = np.ascontiguousarray(wl.c.copy())
-    orpic.apic_step_omp(2, wl.low, wl.h, lo, n, x, wl.m, c, mode=1, nthreads=threads)  # warm-up
-    ts = []
-    for _ in range(reps):
-        c[:] = wl.c
+
+    def run(wlx, xx, nt, keep=False):
+        cc = np.ascontiguousarray(wlx.c.copy())
+        lo_, n_ = wlx.node_window
         t0 = time.perf_counter()
-        orpic.apic_step_omp(2, wl.low, wl.h, lo, n, x, wl.m, c, mode=1, nthreads=threads)
-        ts.append(time.perf_counter() - t0)
+        gm, gmu, _ = orpic.apic_step_omp(2, wlx.low, wlx.h, lo_, n_, xx, wlx.m, cc, mode=1,
+                                         nthreads=nt)
+        t = time.perf_counter() - t0
+        return t, ((gm, gmu, cc) if keep else None)
+
+    run(wl, x, threads)  # warm-up
+    ts, out = [], None
+    for r in range(reps):
+        t, o = run(wl, x, threads, keep=(r == reps - 1))
+        ts.append(t)
+        out = o if o is not None else out
     t = float(np.median(ts))
-    return {"value": wl.np / t, "unit": "particles/s", "cores": threads, "kind": "port",
-            "sample": f"full workload ({wl.np} particles), FP64 p2g(atomic)->normalise->g2p, "
-                      f"median of {reps} after 1 warm-up, {t * 1e3:.1f} ms/step"}
+    small = synth.apic_uniform(32, 8)
+    xs = np.ascontiguousarray(small.x)
+    run(small, xs, 1)
+    t1 = float(np.median([run(small, xs, 1)[0] for _ in range(reps)]))
+    shp = tuple(int(v) for v in n)
+    ref = (out[0].reshape(shp), out[1].reshape(shp + (3,)), out[2].reshape(-1, 4, 3))
+    rec = {"value": wl.np / t, "unit": "particles/s", "cores": threads, "kind": "port",
+           "sample": f"full workload ({wl.np} particles), FP64 p2g(atomic)->normalise->g2p, "
+                     f"{threads} OpenMP threads bound to cores, median of {reps} after 1 "
+                     f"warm-up, {t * 1e3:.1f} ms/step",
+           "cpu_model": model, "physical_cores_on_host": phys, "cpus_available": avail,
+           "one_core": {"value": small.np / t1, "unit": "particles/s",
+                        "sample": f"32^3 x 8 ppc ({small.np} particles), FP64, 1 thread, "
+                                  f"median of {reps}, {t1 * 1e3:.1f} ms/step"}}
+    return rec, ref
 
 
 def _time(fn, steps, warmup):
@@ -417,14 +508,15 @@ def config_mac(pa, ctx, synth, steps=10, steps_particles=3):
     return out
 
 
-def config_batched(pa, ctx, synth, n_inst=512, steps=3, lanes=4):
+def config_batched(pa, ctx, synth, n_inst=512, steps=5):
     """Independent instances: grid edge n_k = round(16 * 4^u_k), u_k ~ U(0, 1)
     (16^3 to 64^3 cells, log-uniform), ppc_k ~ U{2..16}, fields as the 32^3
-    workload, FP32, quadratic node spline.  One step = for every instance:
-    sort -> p2g -> g2p (normalisation fused).  The instances are dealt,
-    largest first, to `lanes` streams with a context each (own scratch); each
-    lane's part of the step is captured once into a step graph
-    (pic_graph_*), and the lanes' graphs are replayed concurrently."""
+    workload, FP32, quadratic node spline.  One step = bin sort -> p2g (grids
+    zeroed first) -> g2p with the normalisation fused, for every instance,
+    through the batch entry points (pic_batch_*): each pass is one launch
+    over all instances' bricks, and the whole step is captured once in a
+    step graph and replayed.  The per-instance spread is the eager time of
+    each instance run alone through the single-instance entry points."""
     import torch
     dev = "cuda:0"
     rs = np.random.default_rng(0x5EED0003)
@@ -439,7 +531,7 @@ def config_batched(pa, ctx, synth, n_inst=512, steps=3, lanes=4):
     M = torch.empty((1, P), dtype=torch.float32, device=dev)
     C = torch.empty((12, P), dtype=torch.float32, device=dev)
     side = torch.cuda.Stream()
-    inst = []
+    inst, single = [], []
     with torch.cuda.stream(side):
         for k in range(n_inst):
             n, ppc, a, b = int(ns[k]), int(ppcs[k]), int(offs[k]), int(offs[k + 1])
@@ -451,100 +543,108 @@ def config_batched(pa, ctx, synth, n_inst=512, steps=3, lanes=4):
             M[:, a:b] = h ** 3 / ppc
             C[0:3, a:b] = torch.randn((3, b - a), generator=gen, device=dev)
             C[3:, a:b] = torch.randn((9, b - a), generator=gen, device=dev) * 0.1
-            low = np.full(3, -synth.HALO * h)
-            mesh = pa.Mesh(low, h)
+            mesh = pa.Mesh(np.full(3, -synth.HALO * h), h)
             geom = pa.Binning((synth.HALO,) * 3, (n + 1,) * 3, (4, 4, 4), 1)
-            xv = X[:, a:b].t()
-            bins = pa.bin_sort(ctx, mesh, xv, geom)
+            bins = pa.bin_sort(ctx, mesh, X[:, a:b].t(), geom)
             ctx.synchronize()  # raises if a particle fell outside the bins
-            perm = bins.perm.long() + a
+            perm = bins.perm.long() + a  # store the instance's particles in bin order
             for t in (X, M, C):
                 t[:, a:b] = t[:, perm]
             nn = n + 1 + 2 * synth.HALO
             gm = pa.GridField(torch.zeros((nn, nn, nn), dtype=torch.float32, device=dev))
             gmu = pa.GridField(torch.zeros((3, nn, nn, nn), dtype=torch.float32,
                                            device=dev).permute(1, 2, 3, 0))
-            inst.append(dict(mesh=mesh, geom=geom, bins=bins, x=X[:, a:b].t(), m=M[:, a:b].t(),
-                             c=C[:, a:b].t(), gm=gm, gmu=gmu, np=b - a))
+            inst.append(pa.Instance(mesh, a, b - a, geom, gm, gmu))
+            single.append(dict(mesh=mesh, geom=geom, bins=bins, x=X[:, a:b].t(),
+                               m=M[:, a:b].t(), c=C[:, a:b].t(), gm=gm, gmu=gmu))
         C0 = C.clone()
-    torch.cuda.synchronize()
-
-    def run(cx, d):
-        pa.bin_sort(cx, d["mesh"], d["x"], d["geom"], out=d["bins"])
-        sd = pa.create_spline(pa.FieldLocation.Node, 2, d["mesh"], d["x"], d["bins"])
-        pa.APIC.p2g(cx, d["m"], d["c"], d["gm"], d["gmu"], sd, zero_first=True)
-        pa.APIC.g2p_normalized(cx, d["gm"], d["gmu"], d["c"], sd)
-
-    # Eager pass on one stream: times every instance on its own.
-    with torch.cuda.stream(side):
+        # Per-instance spread: each instance alone through the single-instance
+        # entry points, eagerly.
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in inst]
-        for d, (e0, e1) in zip(inst, ev):
+              for _ in single]
+        for d, (e0, e1) in zip(single, ev):
             e0.record()
-            run(ctx, d)
+            pa.bin_sort(ctx, d["mesh"], d["x"], d["geom"], out=d["bins"])
+            sd = pa.create_spline(pa.FieldLocation.Node, 2, d["mesh"], d["x"], d["bins"])
+            pa.APIC.p2g(ctx, d["m"], d["c"], d["gm"], d["gmu"], sd, zero_first=True)
+            pa.APIC.g2p_normalized(ctx, d["gm"], d["gmu"], d["c"], sd)
             e1.record()
         ctx.synchronize()
         per_inst = np.array([e0.elapsed_time(e1) for e0, e1 in ev])
-    # Deal instances to lanes, largest first, to the least loaded lane.
-    order = np.argsort(-counts)
-    load = np.zeros(lanes)
-    groups = [[] for _ in range(lanes)]
-    for k in order:
-        j = int(np.argmin(load))
-        groups[j].append(int(k))
-        load[j] += counts[k]
-    streams = [torch.cuda.Stream() for _ in range(lanes)]
-    ctxs = [pa.Context(0) for _ in range(lanes)]
-    # Eager pass per lane: sizes each context's scratch before the capture.
-    for cx, st, grp in zip(ctxs, streams, groups):
-        with torch.cuda.stream(st):
-            for k in grp:
-                run(cx, inst[k])
-    torch.cuda.synchronize()
-    C.copy_(C0)
-    torch.cuda.synchronize()
-    # One step graph per lane, replayed on the lane's own stream so the
-    # lanes run concurrently.
-    t0 = time.perf_counter()
-    graphs = []
-    for cx, st, grp in zip(ctxs, streams, groups):
-        with torch.cuda.stream(st):
-            cx.graph_begin()
-            for k in grp:
-                run(cx, inst[k])
-            graphs.append(cx.graph_end())
-    t_capture = time.perf_counter() - t0
-
-    def replay():
-        for g_, st in zip(graphs, streams):
-            with torch.cuda.stream(st):
-                g_.launch()
-
-    replay()  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        replay()
-        for st in streams:  # a step ends when every lane has finished
-            side.wait_stream(st)
-        for st in streams:
-            st.wait_stream(side)
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / steps
-    for cx, g_ in zip(ctxs, graphs):
-        cx.synchronize()
-        g_.close()
-        cx.close()
+        sequential_ms = float(per_inst.sum())
+        C.copy_(C0)
+        # The batch: one launch per pass over every instance, captured once.
+        batch = pa.Batch(ctx, 2, torch.float32, inst)
+        xs, ms_, cs = X.t(), M.t(), C.t()
+        batch.step(ctx, xs, ms_, cs)  # eager warm-up
+        ctx.synchronize()
+        C.copy_(C0)
+        t0 = time.perf_counter()
+        ctx.graph_begin()
+        batch.step(ctx, xs, ms_, cs)
+        graph = ctx.graph_end()
+        t_capture = time.perf_counter() - t0
+        graph.launch()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            graph.launch()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        ctx.synchronize()
+        # kernel families of one eager step
+        ctx.reset_timing()
+        ctx.set_timing(True)
+        batch.step(ctx, xs, ms_, cs)
+        ctx.set_timing(False)
+        ctx.synchronize()
+        kt = _kernel_avgs(ctx, ["batch_sort", "batch_zero", "batch_p2g", "batch_p2g_stale",
+                                "batch_g2p", "batch_g2p_stale"])
+        ctx.reset_timing()
+        graph.close()
+        batch.close()
+    n_touched = int(((ns + 3) ** 3).sum())  # upper bound: every node of each window's cells
+    alg = P * 31 * 4 + n_touched * 14 * 4
     return {"workload": f"{n_inst} APIC instances, 16^3-64^3 cells (log-uniform), 2-16 ppc, FP32, "
-                        f"quadratic node spline, sort -> p2g -> g2p per instance, {lanes} concurrent "
-                        "streams with one step graph each",
+                        "quadratic node spline, particles stored in bin order per instance; "
+                        "batched step (sort -> p2g -> normalised g2p, one launch per pass over "
+                        "all instances) replayed as one step graph",
             "instances": n_inst, "particles": P, "ms_per_step": ms,
             "particles_per_s": P / (ms * 1e-3), "graph_capture_s": t_capture,
+            "kernels_ms": kt,
+            "alg_bytes_upper": alg, "alg_GBs_upper": alg / (ms * 1e-3) / 1e9,
+            "sequential_single_instance_ms": sequential_ms,
             "instance_ms_eager": {"min": float(per_inst.min()),
                                   "median": float(np.median(per_inst)),
                                   "max": float(per_inst.max())},
             "instance_particles": {"min": int(counts.min()), "median": int(np.median(counts)),
                                    "max": int(counts.max())}}
+
+
+OTHER_CONFIGS = [
+    ("plumbing", "apic_uniform_32", "plumbing 32^3 FP64", config_plumbing),
+    ("f_update", "apic_cube_f_update", "cube 256^3 FP32 + FP64 F update", config_f_update),
+    ("polypic", "polypic_cubic27", "PolyPIC cubic 27 modes 128^3 FP32", config_polypic),
+    ("interp", "generic_interp", "generic transfers 256^3 FP32", config_interp),
+    ("mac", "apic_mac_faces", "MAC grid, three Face lattices, 256^3 FP32", config_mac),
+    ("batched", "batched_instances", "512 batched instances", config_batched),
+]
+
+
+def run_others(pa, ctx, synth, only=""):
+    """The other workloads of the scope (all, or the comma-separated subset
+    `only` by short name)."""
+    import torch
+    want = {w for w in only.split(",") if w}
+    others = {}
+    for short, key, title, fn in OTHER_CONFIGS:
+        if want and short not in want:
+            continue
+        log(f"config: {title} ...")
+        others[key] = fn(pa, ctx, synth)
+        log(f"  {others[key]}")
+        torch.cuda.empty_cache()
+    return others
 
 
 def main():
@@ -555,10 +655,17 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--ppc", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-oracle-check", action="store_true",
+                    help="skip the comparison of the headline outputs with the FP64 oracle")
     ap.add_argument("--skip-shuffled", action="store_true",
                     help="skip the reference step on shuffled storage (profiling runs)")
     ap.add_argument("--no-other-configs", action="store_true",
                     help="time only the headline workload")
+    ap.add_argument("--only", default="",
+                    help="comma-separated subset of the other configs to time (plumbing, "
+                         "f_update, polypic, interp, mac, batched)")
+    ap.add_argument("--skip-headline", action="store_true",
+                    help="skip the headline workload (with --only, for a config's own run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the transfer path is built and timed on a single "
@@ -569,6 +676,11 @@ def main():
 
     import picasso_amd as pa
     from picasso_amd import synth
+
+    if args.skip_headline:
+        ctx = pa.Context(0)
+        print(json.dumps({"other_configs": run_others(pa, ctx, synth, args.only)}))
+        return
 
     t0 = time.perf_counter()
     wl = synth.apic_sphere(args.n, args.ppc)
@@ -629,6 +741,12 @@ def main():
     log(f"fp64: {ms64_eager:.3f} ms/step eager, kernels {json.dumps(kt64)}")
     ms64 = time_graph_steps(s64, args.steps, args.warmup, ctx)
     log(f"fp64: {ms64:.3f} ms/step as a step graph")
+    gpu_out = {}
+    if not args.no_oracle_check:
+        s64.restore()
+        s64.run()
+        ctx.synchronize()
+        gpu_out["fp64"] = s64.outputs()
     del s64
 
     # Error of the FP32 step against the FP64 step on the same rounded inputs.
@@ -651,6 +769,8 @@ def main():
            "particle_B": rel(s32.c[:, 3:], ref.c[:, 3:])}
     log(f"fp32 error vs fp64: {err}")
     del ref
+    if not args.no_oracle_check:
+        gpu_out["fp32"] = s32.outputs()
 
     s_alg32 = 4
     B_alg32 = P * 31 * s_alg32 + n_touched * 14 * s_alg32
@@ -684,35 +804,25 @@ def main():
     if not args.no_other_configs:
         del s32
         torch.cuda.empty_cache()
-        others = {}
-        log("config: plumbing 32^3 FP64 ...")
-        others["apic_uniform_32"] = config_plumbing(pa, ctx, synth)
-        log(f"  {others['apic_uniform_32']}")
-        log("config: cube 256^3 FP32 + FP64 F update ...")
-        others["apic_cube_f_update"] = config_f_update(pa, ctx, synth)
-        log(f"  {others['apic_cube_f_update']}")
-        log("config: PolyPIC cubic 27 modes 128^3 FP32 ...")
-        others["polypic_cubic27"] = config_polypic(pa, ctx, synth)
-        log(f"  {others['polypic_cubic27']}")
-        torch.cuda.empty_cache()
-        log("config: generic transfers 256^3 FP32 ...")
-        others["generic_interp"] = config_interp(pa, ctx, synth)
-        log(f"  {others['generic_interp']}")
-        torch.cuda.empty_cache()
-        log("config: MAC grid, three Face lattices, 256^3 FP32 ...")
-        others["apic_mac_faces"] = config_mac(pa, ctx, synth)
-        log(f"  {others['apic_mac_faces']}")
-        torch.cuda.empty_cache()
-        log("config: 512 batched instances ...")
-        others["batched_instances"] = config_batched(pa, ctx, synth)
-        log(f"  {others['batched_instances']}")
-        torch.cuda.empty_cache()
+        others = run_others(pa, ctx, synth, args.only)
 
     cpu = None
+    oracle_err = {}
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        cpu = cpu_baseline(wl)
+        cpu, ref64 = cpu_baseline(wl, synth)
         log(f"cpu baseline {cpu}")
+        if "fp64" in gpu_out:
+            oracle_err["fp64"] = field_errors(gpu_out["fp64"], ref64)
+            log(f"fp64 error vs the FP64 oracle: {oracle_err['fp64']}")
+        del ref64
+    if "fp32" in gpu_out:
+        # the FP64 oracle on the FP32-rounded inputs the FP32 step computed from
+        ref32 = oracle_step(wl, x32.astype(np.float64), m32.astype(np.float64),
+                            c32.astype(np.float64))
+        oracle_err["fp32"] = field_errors(gpu_out["fp32"], ref32)
+        log(f"fp32 error vs the FP64 oracle: {oracle_err['fp32']}")
+        del ref32
 
     step_gbs = B_alg32 / (ms32 * 1e-3) / 1e9
     out = {
@@ -742,6 +852,8 @@ def main():
         "fp32_step_ms_shuffled_storage": shuffled_ms,
         "fp32_first_step_ms_from_shuffled": shuffled_first_ms,
         "fp32_error_vs_fp64": err,
+        "fp32_error_vs_oracle": oracle_err.get("fp32"),
+        "fp64_error_vs_oracle": oracle_err.get("fp64"),
         "step_alg_bytes": B_alg32,
         "step_alg_GBs": step_gbs,
         "step_hbm_frac": step_gbs / HBM_PEAK_GBS,

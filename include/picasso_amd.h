@@ -387,6 +387,50 @@ int pic_graph_end(pic_context* ctx, pic_graph** graph);
 int pic_graph_launch(pic_context* ctx, const pic_graph* graph);
 int pic_graph_destroy(pic_graph* graph);
 
+/* Batches of independent instances (no reference counterpart: Picasso runs
+ * one simulation per GridOperator, whose apply launches one Kokkos kernel
+ * per operator, Picasso_GridOperator.hpp:632-641).  An instance is one APIC
+ * simulation on Node splines with its own local mesh, its particles
+ * [first, first + np) of the batch's particle fields (instance k + 1 starts
+ * where instance k ends), its binning (4x4x4 bricks on the spline's knot
+ * lattice: dual 1 for quadratic splines, 0 for linear and cubic) and its
+ * grid windows: mass m_i, and momentum mu_i with 3 components over the same
+ * window, each window one dense range of memory.  pic_batch_create checks
+ * and uploads the instance table and allocates everything the batch's calls
+ * use (bins, walk lists, scratch), so those calls allocate nothing and a
+ * whole step can be captured in one step graph.  Each call is one launch
+ * per pass over the bricks of every instance.  Errors detected on the
+ * device are reported by pic_synchronize as for the other calls. */
+typedef struct pic_instance {
+    pic_mesh mesh;
+    int64_t first;
+    int64_t np;
+    pic_binning geom;
+    pic_grid m_i;
+    pic_grid mu_i;
+} pic_instance;
+
+typedef struct pic_batch pic_batch;
+int pic_batch_create(pic_context* ctx, int order, int dtype, int ninst,
+                     const pic_instance* inst, pic_batch** batch);
+int pic_batch_destroy(pic_batch* batch);
+/* pic_bin_sort of every instance's particles by its own bins, in one pass
+ * over all particles.  x holds the particles of every instance. */
+int pic_batch_bin_sort(pic_context* ctx, pic_batch* batch, const pic_pfield* x);
+/* APIC::p2g (Picasso_APIC.hpp:55-140, 240-319) of every instance into its
+ * own grids through the bins of the last pic_batch_bin_sort;
+ * PIC_P2G_ZERO_FIRST zeroes every instance's windows first. */
+int pic_batch_apic_p2g(pic_context* ctx, pic_batch* batch, const pic_pfield* x,
+                       const pic_pfield* m_p, const pic_pfield* c_p, uint32_t flags);
+/* APIC::g2p (Picasso_APIC.hpp:407-472) of every instance from its grid
+ * velocity mu_i / m_i formed on the fly, as pic_apic_g2p_normalized. */
+int pic_batch_apic_g2p_normalized(pic_context* ctx, pic_batch* batch, const pic_pfield* x,
+                                  const pic_pfield* c_p);
+/* One APIC step of every instance: pic_batch_bin_sort, pic_batch_apic_p2g,
+ * pic_batch_apic_g2p_normalized. */
+int pic_batch_apic_step(pic_context* ctx, pic_batch* batch, const pic_pfield* x,
+                        const pic_pfield* m_p, const pic_pfield* c_p, uint32_t flags);
+
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on
  * the context's stream and pic_kernel_time() returns the accumulated
  * milliseconds and launch count of the named kernel family

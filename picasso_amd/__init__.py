@@ -30,7 +30,7 @@ __all__ = ["Context", "Mesh", "GridField", "create_spline", "APIC", "PolyPIC", "
            "grid_normalize", "g2p_gradu", "update_F", "bin_sort", "bin_count", "Binning",
            "BinSort", "PicError", "DUAL_DIMS", "aligned_dual", "InitUniform", "InitRandom",
            "initialize_particles", "select", "filter_particles", "halo_scatter",
-           "halo_gather", "AoSoA", "AoSoASlice", "permute", "G2P", "P2G",
+           "halo_gather", "AoSoA", "AoSoASlice", "permute", "G2P", "P2G", "Instance", "Batch",
            "lib", "LIB_PATH"]
 
 LIB_PATH = _abi.LIB_PATH
@@ -647,6 +647,83 @@ def permute(ctx, perm, pairs):
     rc = lib.pic_permute(ctx._bind_stream(), dt, np_, ctypes.c_void_p(perm.data_ptr()), n,
                          srcs, dsts, nc)
     check(rc, "pic_permute")
+
+
+@dataclass
+class Instance:
+    """One independent APIC simulation of a Batch: its mesh, its particles
+    [first, first + np) of the batch's particle fields, its bins (4x4x4
+    bricks on the spline's knot lattice) and its grid windows (mass, and
+    3-component momentum over the same window)."""
+    mesh: Mesh
+    first: int
+    np: int
+    geom: Binning
+    m_i: GridField
+    mu_i: GridField
+
+
+class Batch:
+    """Many independent APIC instances stepped together (pic_batch_*): every
+    pass is one launch over all instances' bricks, and the batch owns its
+    bins and work buffers, so its calls allocate nothing and a step can be
+    captured in a step graph."""
+
+    def __init__(self, ctx, order, dtype, instances):
+        import torch
+        self.dt = F64 if dtype == torch.float64 else F32
+        self.np = sum(int(i.np) for i in instances)
+        self.instances = list(instances)  # keeps the grid tensors alive
+        tab = (_abi.Instance * len(instances))()
+        for k, inst in enumerate(instances):
+            for g, nm in ((inst.m_i, "m_i"), (inst.mu_i, "mu_i")):
+                _grid(g, nm, self.dt, 1 if nm == "m_i" else 3)
+            tab[k] = _abi.Instance(inst.mesh.c(), int(inst.first), int(inst.np), inst.geom.c(),
+                                   inst.m_i.c(), inst.mu_i.c())
+        h = ctypes.c_void_p()
+        check(lib.pic_batch_create(ctx._bind_stream(), int(order), self.dt, len(instances), tab,
+                                   ctypes.byref(h)), "pic_batch_create")
+        self.handle = h
+
+    def _fields(self, x, m_p, c_p):
+        out = [ctypes.byref(_pfield(x, 3, "x", self.np, self.dt))]
+        if m_p is not None:
+            out.append(ctypes.byref(_pfield(m_p, 1, "m_p", self.np, self.dt)))
+        out.append(ctypes.byref(_pfield(c_p, 12, "c_p", self.np, self.dt)))
+        return out
+
+    def bin_sort(self, ctx, x):
+        xf = ctypes.byref(_pfield(x, 3, "x", self.np, self.dt))
+        check(lib.pic_batch_bin_sort(ctx._bind_stream(), self.handle, xf), "pic_batch_bin_sort")
+
+    def p2g(self, ctx, x, m_p, c_p, zero_first=True):
+        f = self._fields(x, m_p, c_p)
+        check(lib.pic_batch_apic_p2g(ctx._bind_stream(), self.handle, *f,
+                                     _abi.P2G_ZERO_FIRST if zero_first else 0),
+              "pic_batch_apic_p2g")
+
+    def g2p_normalized(self, ctx, x, c_p):
+        f = self._fields(x, None, c_p)
+        check(lib.pic_batch_apic_g2p_normalized(ctx._bind_stream(), self.handle, *f),
+              "pic_batch_apic_g2p_normalized")
+
+    def step(self, ctx, x, m_p, c_p, zero_first=True):
+        """Sort, p2g and normalised g2p of every instance."""
+        f = self._fields(x, m_p, c_p)
+        check(lib.pic_batch_apic_step(ctx._bind_stream(), self.handle, *f,
+                                      _abi.P2G_ZERO_FIRST if zero_first else 0),
+              "pic_batch_apic_step")
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib.pic_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 InitUniform, InitRandom = 0, 1  # Picasso_ParticleInit.hpp:30-35 (tags)

@@ -50,6 +50,11 @@ class Bins(ctypes.Structure):
                 ("geom", Binning)]
 
 
+class Instance(ctypes.Structure):
+    _fields_ = [("mesh", Mesh), ("first", ctypes.c_int64), ("np", ctypes.c_int64),
+                ("geom", Binning), ("m_i", Grid), ("mu_i", Grid)]
+
+
 _P = ctypes.POINTER
 _c_int, _c_i64, _c_u32, _c_double, _vp = (ctypes.c_int, ctypes.c_int64, ctypes.c_uint32,
                                           ctypes.c_double, ctypes.c_void_p)
@@ -101,6 +106,12 @@ PROTOTYPES = {
     "pic_select": (_c_int, [_vp, _c_i64, _vp, _vp]),
     "pic_grid_halo": (_c_int, [_vp, _c_int, _c_int, _P(Grid), _c_int, _P(ctypes.c_int32),
                                _P(ctypes.c_int32), _P(ctypes.c_int32)]),
+    "pic_batch_create": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(Instance), _P(_vp)]),
+    "pic_batch_destroy": (_c_int, [_vp]),
+    "pic_batch_bin_sort": (_c_int, [_vp, _vp, _P(PField)]),
+    "pic_batch_apic_p2g": (_c_int, [_vp, _vp, _P(PField), _P(PField), _P(PField), _c_u32]),
+    "pic_batch_apic_g2p_normalized": (_c_int, [_vp, _vp, _P(PField), _P(PField)]),
+    "pic_batch_apic_step": (_c_int, [_vp, _vp, _P(PField), _P(PField), _P(PField), _c_u32]),
     "pic_graph_begin": (_c_int, [_vp]),
     "pic_graph_end": (_c_int, [_vp, _P(_vp)]),
     "pic_graph_launch": (_c_int, [_vp, _vp]),

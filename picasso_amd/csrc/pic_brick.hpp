@@ -26,6 +26,7 @@ struct alignas(16) BrickItem {
     uint32_t* walk;         /* WALK_LANES counters, WALK_PAD words apart, zero */    \
     uint32_t* stale;        /* [count, slots...] of particles for the slow path */   \
     uint32_t nbricks;                                                                \
+    uint32_t brick_base; /* brick id of bin brick 0 (batches: the instance's first) */ \
     int32_t cell_lo[3];                                                              \
     int32_t nb[3];                                                                   \
     int32_t knot_off[3]; /* first knot index = bin index + knot_off, per dimension */
@@ -43,6 +44,7 @@ constexpr uint32_t NUM_XCD = 8;  // accelerator complex dies of an MI355X, one L
 template <int BB, typename G>
 __device__ __forceinline__ void brick_origin(const G& g, uint32_t b, int f0[3])
 {
+    b -= g.brick_base;
     const int bk = int(b % uint32_t(g.nb[2]));
     const int bj = int((b / uint32_t(g.nb[2])) % uint32_t(g.nb[1]));
     const int bi = int(b / (uint32_t(g.nb[2]) * uint32_t(g.nb[1])));
@@ -383,6 +385,7 @@ inline int setup_walk(pic_context* ctx, int order, int entity, int64_t np, const
     g.bin_start = bins.bin_start;
     g.active = bins.active;
     g.nbricks = uint32_t(nb);
+    g.brick_base = 0;
     for (int d = 0; d < 3; ++d) {
         g.cell_lo[d] = bins.geom.cell_lo[d];
         g.nb[d] = (bins.geom.ncell[d] + bb - 1) / bb;

@@ -41,22 +41,6 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
         p[i] = v;
 }
 
-// Wave-aggregated count: atomicAdd of 1 per active lane on base[key], lanes
-// with equal keys sharing one atomic.  Nothing is returned, so the atomics
-// are fire-and-forget and a work-item's later items do not wait for them.
-__device__ __forceinline__ void wave_agg_count(uint32_t* base, uint32_t key, bool active)
-{
-    const int lane = __lane_id();
-    uint64_t pending = __ballot(active);
-    while (pending) {
-        const int leader = __ffsll((unsigned long long)pending) - 1;
-        const uint32_t lkey = __shfl(key, leader);
-        const uint64_t same = __ballot(active && key == lkey) & pending;
-        if (lane == leader) (void)atomicAdd(base + lkey, (uint32_t)__popcll(same));
-        pending &= ~same;
-    }
-}
-
 // Wave-aggregated slot claim in two phases, for several calls in flight:
 // issue() does the aggregated atomics and keeps, per lane, its group's
 // leader and its rank (among equal-key lanes, in lane order); resolve()
@@ -92,66 +76,8 @@ __device__ __forceinline__ uint32_t wave_agg_resolve(const AggTicket& t)
     return uint32_t(__shfl((int)t.base, t.leader)) + t.rank;
 }
 
-// Particles per work-item in the per-particle sort passes (the grid is a
-// quarter of the particles).  The key pass waits for each item's position
-// before loading the next; at 40 registers, eight waves per SIMD hide that
-// (issuing all twelve loads first measured no faster).
-constexpr int SORT_ITEMS = 4;
-constexpr int SORT_PT = 256;  // work-items per block of those passes
-
-// Particles outside the bins are counted in count[nbricks] (they are placed
-// after every brick, so perm stays a permutation even then).  Keys go to
-// keys[] in input order.  *unsorted is raised (by one atomic per
-// workgroup) when some particle's key is smaller than its predecessor's;
-// the predecessor of a workgroup's first particle is recomputed from its
-// position.
-template <typename T>
-__global__ __launch_bounds__(SORT_PT) void k_cell_keys(int64_t np, PField<T> x, KeyGeom g,
-                                                       uint32_t nbricks, uint32_t* keys,
-                                                       uint32_t* count, uint32_t* unsorted,
-                                                       int* err)
-{
-    __shared__ uint32_t skey[SORT_PT * SORT_ITEMS];
-    const int64_t blk0 = int64_t(blockIdx.x) * (SORT_PT * SORT_ITEMS);
-    const int64_t p0 = blk0 + threadIdx.x;
-    uint32_t key[SORT_ITEMS], b[SORT_ITEMS];
-#pragma unroll
-    for (int k = 0; k < SORT_ITEMS; ++k) {
-        const int64_t p = p0 + k * SORT_PT;
-        key[k] = 0xffffffffu;
-        b[k] = 0;
-        if (p < np) key[k] = cell_key(g, x, p, &b[k]);
-    }
-    uint32_t before = 0u;  // key of the workgroup's predecessor particle
-    if (threadIdx.x == 0 && blk0 > 0) {
-        uint32_t bb;
-        before = cell_key(g, x, blk0 - 1, &bb);
-    }
-#pragma unroll
-    for (int k = 0; k < SORT_ITEMS; ++k) {
-        const int64_t p = p0 + k * SORT_PT;
-        const bool live = p < np;
-        skey[k * SORT_PT + threadIdx.x] = key[k];
-        if (live) {
-            keys[p] = key[k];
-            if (key[k] == 0xffffffffu) {
-                atomicOr(err, ERRBIT_CELL);
-                b[k] = nbricks;
-            }
-        }
-        wave_agg_count(count, b[k], live);
-    }
-    __syncthreads();
-    bool descent = false;
-#pragma unroll
-    for (int k = 0; k < SORT_ITEMS; ++k) {
-        const int q = k * SORT_PT + int(threadIdx.x);
-        if (blk0 + q >= np) continue;
-        const uint32_t prev = q > 0 ? skey[q - 1] : before;
-        descent = descent || key[k] < prev;
-    }
-    if (__syncthreads_or(descent) && threadIdx.x == 0) atomicOr(unsorted, 1u);
-}
+// Particles per work-item and work-items per block of the per-particle
+// passes (SORT_ITEMS, SORT_PT) are in pic_sort.hpp with the key pass.
 
 // ---------------------------------------------------------------------------
 // bin_start (exclusive scan of the nb + 1 counts) and the list of non-empty
@@ -531,26 +457,6 @@ KeyGeom make_key_geom(const pic_mesh& mesh, const pic_binning& b)
 
 namespace pic {
 
-// Scratch of one sort: count[nb+1] | cursor[nb+1] | unsorted flag (2 words)
-// | tile sums (64-bit) | act[nb+1] | tmp[np] | tmpk[np] | big[np], in
-// 4-byte words.  act holds the active-brick list when the caller does not
-// ask for it.  The first nzero words must start at zero (the first pass
-// clears them).
-struct SortLayout {
-    int64_t ntile, nzero, head;
-    size_t bytes;
-};
-
-static SortLayout sort_layout(int64_t nb, int64_t np, bool caller_active)
-{
-    SortLayout l;
-    l.ntile = (nb + 1 + SCAN_TILE - 1) / SCAN_TILE;
-    l.nzero = 2 * (nb + 1) + 2;
-    l.head = l.nzero + 2 * l.ntile + (caller_active ? 0 : nb + 1);
-    l.bytes = size_t(l.head + 3 * np) * 4;
-    return l;
-}
-
 static int check_sort_args(pic_context* ctx, int dtype, const pic_mesh* mesh, int64_t np,
                            const pic_pfield* x, const pic_binning* geom, const uint32_t* keys,
                            const uint32_t* perm, const uint32_t* bin_start, int64_t* nb)
@@ -563,14 +469,26 @@ static int check_sort_args(pic_context* ctx, int dtype, const pic_mesh* mesh, in
     return *nb < 0 ? int(*nb) : PIC_OK;
 }
 
-// The sort's passes on the context's stream with scratch `sc` laid out by
-// sort_layout (device memory of at least L.bytes).
-static int bin_sort_run(pic_context* ctx, int dtype, const pic_mesh& mesh, int64_t np,
-                        const pic_pfield* x, const pic_binning& geom, int64_t nb,
-                        const SortLayout& L, uint32_t* sc, uint32_t* keys, uint32_t* perm,
-                        uint32_t* bin_start, uint32_t* active)
+SortLayout sort_layout(int64_t nb, int64_t np, bool caller_active)
 {
-    const KeyGeom g = make_key_geom(mesh, geom);
+    SortLayout l;
+    l.ntile = (nb + 1 + SCAN_TILE - 1) / SCAN_TILE;
+    l.nzero = 2 * (nb + 1) + 2;
+    l.head = l.nzero + 2 * l.ntile + (caller_active ? 0 : nb + 1);
+    l.bytes = size_t(l.head + 3 * np) * 4;
+    return l;
+}
+
+void sort_clear(pic_context* ctx, const SortLayout& L, uint32_t* sc)
+{
+    const unsigned fb = (unsigned)std::min<int64_t>((L.nzero + 255) / 256, 4096);
+    k_fill_u32<<<fb, 256, 0, ctx->stream>>>(sc, L.nzero, 0u);
+}
+
+int sort_finish(pic_context* ctx, int64_t np, int64_t nb, uint32_t bvol, int32_t vshift,
+                const SortLayout& L, uint32_t* sc, uint32_t* keys, uint32_t* perm,
+                uint32_t* bin_start, uint32_t* active)
+{
     uint32_t* count = sc;
     uint32_t* cursor = count + nb + 1;
     uint32_t* unsorted = cursor + nb + 1;
@@ -580,18 +498,6 @@ static int bin_sort_run(pic_context* ctx, int dtype, const pic_mesh& mesh, int64
     uint32_t* tmpk = tmp + np;
     uint32_t* big = tmpk + np;
     hipStream_t st = ctx->stream;
-
-    const unsigned fb = (unsigned)std::min<int64_t>((L.nzero + 255) / 256, 4096);
-    k_fill_u32<<<fb, 256, 0, st>>>(count, L.nzero, 0u);
-    if (np > 0) {
-        const unsigned b = blocks_for(np, SORT_PT * SORT_ITEMS);
-        if (dtype == PIC_F64)
-            k_cell_keys<double><<<b, SORT_PT, 0, st>>>(np, make_pfield<double>(*x), g, uint32_t(nb),
-                                                    keys, count, unsorted, ctx->d_err);
-        else
-            k_cell_keys<float><<<b, SORT_PT, 0, st>>>(np, make_pfield<float>(*x), g, uint32_t(nb),
-                                                   keys, count, unsorted, ctx->d_err);
-    }
     k_tile_sums<<<(unsigned)L.ntile, SCAN_THREADS, 0, st>>>(count, nb, tile_sums);
     const bool prefixed = L.ntile > SCAN_TILE;
     if (prefixed) k_scan_sums<<<1, SCAN_THREADS, 0, st>>>(tile_sums, L.ntile);
@@ -599,14 +505,38 @@ static int bin_sort_run(pic_context* ctx, int dtype, const pic_mesh& mesh, int64
                                                                bin_start, act);
     if (np > 0) {
         k_bin_scatter<<<blocks_for(np, SORT_PT * SORT_ITEMS), SORT_PT, 0, st>>>(
-            np, keys, g.vshift, g.bvol, uint32_t(nb), bin_start, unsorted, cursor, tmp, tmpk,
-            perm);
-        const size_t lds = size_t(2 * g.bvol + 2 * SORT_LDS_CAP) * 4;
+            np, keys, vshift, bvol, uint32_t(nb), bin_start, unsorted, cursor, tmp, tmpk, perm);
+        const size_t lds = size_t(2 * bvol + 2 * SORT_LDS_CAP) * 4;
         const unsigned nbk = persistent_blocks(ctx, k_brick_sort, SORT_BT, lds, nb + 1);
         k_brick_sort<<<nbk, SORT_BT, lds, st>>>(np, uint32_t(nb), bin_start, act, unsorted, tmp,
-                                                tmpk, g.bvol, big, perm, keys);
+                                                tmpk, bvol, big, perm, keys);
     }
     return launch_status();
+}
+
+// The sort's passes on the context's stream with scratch `sc` laid out by
+// sort_layout (device memory of at least L.bytes).
+static int bin_sort_run(pic_context* ctx, int dtype, const pic_mesh& mesh, int64_t np,
+                        const pic_pfield* x, const pic_binning& geom, int64_t nb,
+                        const SortLayout& L, uint32_t* sc, uint32_t* keys, uint32_t* perm,
+                        uint32_t* bin_start, uint32_t* active)
+{
+    const KeyGeom g = make_key_geom(mesh, geom);
+    sort_clear(ctx, L, sc);
+    if (np > 0) {
+        const unsigned b = blocks_for(np, SORT_PT * SORT_ITEMS);
+        uint32_t* count = sort_count(sc);
+        uint32_t* unsorted = sort_unsorted(sc, nb);
+        if (dtype == PIC_F64)
+            k_cell_keys<<<b, SORT_PT, 0, ctx->stream>>>(
+                np, MeshKeys<double>{make_pfield<double>(*x), g}, uint32_t(nb), keys, count,
+                unsorted, ctx->d_err);
+        else
+            k_cell_keys<<<b, SORT_PT, 0, ctx->stream>>>(
+                np, MeshKeys<float>{make_pfield<float>(*x), g}, uint32_t(nb), keys, count,
+                unsorted, ctx->d_err);
+    }
+    return sort_finish(ctx, np, nb, g.bvol, g.vshift, L, sc, keys, perm, bin_start, active);
 }
 
 }  // namespace pic
