@@ -29,6 +29,10 @@ sys.path.insert(0, ROOT)
 # these when it starts, before torch is imported.
 os.environ.setdefault("OMP_PROC_BIND", "close")
 os.environ.setdefault("OMP_PLACES", "cores")
+try:  # read before the OpenMP runtime binds this thread to one core
+    CPUS_AVAILABLE = len(os.sched_getaffinity(0))
+except AttributeError:
+    CPUS_AVAILABLE = os.cpu_count()
 
 METRIC = ("APIC p2g+g2p particles/s @256³×8ppc, FP32-comp vs FP64 at stated err; "
           "HBM GB/s")
@@ -186,11 +190,7 @@ def _cpu_info():
             cores.add((phys, core))
     except OSError:
         pass
-    try:
-        avail = len(os.sched_getaffinity(0))
-    except AttributeError:
-        avail = os.cpu_count()
-    return model, len(cores) or None, avail
+    return model, len(cores) or None, CPUS_AVAILABLE
 
 
 def oracle_step(wl, x, m, c, threads=0):
@@ -621,6 +621,62 @@ def config_batched(pa, ctx, synth, n_inst=512, steps=5):
                                    "max": int(counts.max())}}
 
 
+def config_moving(pa, ctx, synth, steps=32):
+    """The 256^3 sphere in FP32 with moving particles (B_p = 0 at the start):
+    every step advects x += dt v (dt = h / 2, v from the g2p), so bins go
+    stale and storage order decays.  The tuner (picasso_amd/tune.py) times every combination
+    of re-sort interval, re-permute interval and g2p brick edge over short
+    windows of live steps and keeps the fastest; `steps` further steps are
+    then timed with it.  One step = [sort] -> p2g (zero first) -> normalised
+    g2p -> advection [-> sort + permute into bin order]."""
+    import torch
+    from picasso_amd import tune
+    wl = synth.apic_sphere(256, 8)
+    P = wl.np
+    dev = "cuda:0"
+    x = torch.as_tensor(np.ascontiguousarray(wl.x.astype(np.float32).T), device=dev).t()
+    m = torch.as_tensor(wl.m.astype(np.float32)[None, :], device=dev).t()
+    # B_p starts at zero: the synthetic N(0, 0.1) entries are no velocity
+    # gradient, and their affine term D^-1 B d ~ 0.4 / h would dominate the
+    # velocity of nodes few particles reach; advected, it would carry those
+    # particles out of the domain.  The first g2p makes B consistent.
+    c0 = wl.c.astype(np.float32).copy()
+    c0[:, 1:, :] = 0.0
+    c = torch.as_tensor(np.ascontiguousarray(c0.reshape(P, 12).T), device=dev).t()
+    mesh = pa.Mesh(wl.low, wl.h)
+    cell_lo, ncell = wl.cell_region
+    lo_c, n_c = tuple(int(v) for v in cell_lo), tuple(int(v) + 1 for v in ncell)
+    geom = pa.Binning(lo_c, n_c, (4, 4, 4), 1)
+    geom8 = pa.Binning(lo_c, n_c, (8, 8, 8), 1)
+    lo, n = wl.node_window
+    nn = tuple(int(v) for v in n)
+    gm = pa.GridField(torch.zeros(nn, device=dev))
+    gmu = pa.GridField(torch.zeros((3,) + nn, device=dev).permute(1, 2, 3, 0))
+    step = tune.MovingStep(pa, ctx, mesh, geom, geom8, x, m, c, gm, gmu, 0.5 * wl.h)
+    step.config = tune.StepConfig(1, 1, 4)
+    step.run()  # stores the particles in bin order once
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    best, table = tune.StepTuner().tune(step)
+    t_tune = time.perf_counter() - t0
+    step.step_no = 0
+    ctx.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        step.run()
+    e1.record()
+    torch.cuda.synchronize()
+    ctx.synchronize()  # raises if a particle left the bins or a window
+    ms = e0.elapsed_time(e1) / steps
+    return {"workload": "APIC sphere 256^3 x 8 ppc, FP32, moving particles (x += dt v, "
+                        "dt = h/2): [sort] -> p2g -> normalised g2p -> advect [-> permute], "
+                        f"{steps} steps with the tuned configuration",
+            "particles": P, "ms_per_step": ms, "particles_per_s": P / (ms * 1e-3),
+            "tuned": best.as_dict(), "tuning_s": t_tune, "candidates": table}
+
+
 OTHER_CONFIGS = [
     ("plumbing", "apic_uniform_32", "plumbing 32^3 FP64", config_plumbing),
     ("f_update", "apic_cube_f_update", "cube 256^3 FP32 + FP64 F update", config_f_update),
@@ -628,6 +684,7 @@ OTHER_CONFIGS = [
     ("interp", "generic_interp", "generic transfers 256^3 FP32", config_interp),
     ("mac", "apic_mac_faces", "MAC grid, three Face lattices, 256^3 FP32", config_mac),
     ("batched", "batched_instances", "512 batched instances", config_batched),
+    ("moving", "apic_moving_tuned", "moving particles 256^3 FP32, autotuned", config_moving),
 ]
 
 
@@ -663,7 +720,7 @@ def main():
                     help="time only the headline workload")
     ap.add_argument("--only", default="",
                     help="comma-separated subset of the other configs to time (plumbing, "
-                         "f_update, polypic, interp, mac, batched)")
+                         "f_update, polypic, interp, mac, batched, moving)")
     ap.add_argument("--skip-headline", action="store_true",
                     help="skip the headline workload (with --only, for a config's own run)")
     args = ap.parse_args()
@@ -715,10 +772,10 @@ def main():
         shuffled_ms = (time.perf_counter() - t_sh) * 1e3
         s32.restore()
         log(f"fp32 step on shuffled storage, read through the permutation: {shuffled_ms:.3f} ms")
-        keep = (s32.x, s32.m, s32.c, s32.c0)
+        keep = (s32.x, s32.m, s32.c, s32.c0, s32.order)
         s32.reorder()
         s32.run()
-        s32.x, s32.m, s32.c, s32.c0 = keep
+        s32.x, s32.m, s32.c, s32.c0, s32.order = keep
         s32.restore()
         torch.cuda.synchronize()
         t_sh = time.perf_counter()
@@ -726,7 +783,7 @@ def main():
         s32.run()
         torch.cuda.synchronize()
         shuffled_first_ms = (time.perf_counter() - t_sh) * 1e3
-        s32.x, s32.m, s32.c, s32.c0 = keep
+        s32.x, s32.m, s32.c, s32.c0, s32.order = keep
         log(f"fp32 first step from shuffled storage (sort, permute, step): "
             f"{shuffled_first_ms:.3f} ms")
     s32.reorder()

@@ -244,6 +244,29 @@ enum { PIC_HALO_GATHER = 0, PIC_HALO_SCATTER_SUM = 1 };
 int pic_grid_halo(pic_context* ctx, int op, int dtype, const pic_grid* g, int ncomp,
                   const int32_t own_lo[3], const int32_t own_n[3], const int32_t periodic[3]);
 
+/* GridOperator's apply for a particle operation on one GPU
+ * (Picasso_GridOperator.hpp:459-496 applyImpl).  A C ABI cannot carry the
+ * reference's functor, so the operation is a host callback that issues the
+ * transfer calls (APIC / PolyPIC / G2P / P2G) on the context's stream.  In
+ * order: every gather field's halo is gathered (pic_grid_halo GATHER over
+ * its periodic dimensions, :465-466); every scatter field is zeroed over its
+ * window (:532-533); body(ctx, user) runs (:486-487); every scatter field's
+ * halo is scatter-summed (:489-491).  A field without periodic dimensions
+ * exchanges nothing (no neighbour on one GPU).  A non-zero return of body
+ * stops the apply and is returned. */
+typedef struct pic_field_dep {
+    pic_grid grid;        /* the field's window, ghost layers included */
+    int32_t ncomp;
+    int32_t own_lo[3];    /* its owned entities, as for pic_grid_halo */
+    int32_t own_n[3];
+    int32_t periodic[3];
+} pic_field_dep;
+typedef int (*pic_apply_fn)(pic_context* ctx, void* user);
+int pic_grid_operator_apply(pic_context* ctx, int dtype, int ngather,
+                            const pic_field_dep* gather, int nscatter,
+                            const pic_field_dep* scatter, pic_apply_fn body,
+                            void* user);
+
 /* Grid velocity u_i = mu_i / m_i over the window of m_i (no reference
  * function; doc/basic/explict_nearly_incompressible_flow.md:18-23 without
  * forces).  Nodes with m_i == 0 get u_i = 0.  For PIC_F32, nodes with

@@ -23,8 +23,11 @@
 #ifndef PICASSO_AMD_HPP
 #define PICASSO_AMD_HPP
 
+#include <exception>
 #include <stdexcept>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "picasso_amd.h"
 
@@ -270,6 +273,44 @@ inline void select(pic_context* ctx, int64_t np, const uint8_t* keep, uint32_t* 
 {
     check(pic_select(ctx, np, keep, kept));
 }
+
+// GridOperator::apply for particle operations on one GPU
+// (Picasso_GridOperator.hpp:459-496): gather halos of the gather fields,
+// zeroed scatter fields, op(ctx) issuing the transfers, scatter-sum halos.
+class GridOperator {
+  public:
+    GridOperator(std::vector<pic_field_dep> gather, std::vector<pic_field_dep> scatter)
+        : gather_(std::move(gather)), scatter_(std::move(scatter))
+    {
+    }
+    template <class Op>
+    void apply(pic_context* ctx, int dtype, Op&& op) const
+    {
+        struct Call {
+            Op* op;
+            std::exception_ptr err;
+            static int run(pic_context* c, void* self)
+            {
+                Call* me = static_cast<Call*>(self);
+                try {
+                    (*me->op)(c);
+                    return PIC_OK;
+                } catch (...) {
+                    me->err = std::current_exception();
+                    return PIC_ERR_ARG;
+                }
+            }
+        } call{&op, nullptr};
+        const int rc = pic_grid_operator_apply(ctx, dtype, int(gather_.size()), gather_.data(),
+                                               int(scatter_.size()), scatter_.data(), &Call::run,
+                                               &call);
+        if (call.err) std::rethrow_exception(call.err);
+        check(rc);
+    }
+
+  private:
+    std::vector<pic_field_dep> gather_, scatter_;
+};
 
 namespace Halo {
 

@@ -30,7 +30,7 @@ __all__ = ["Context", "Mesh", "GridField", "create_spline", "APIC", "PolyPIC", "
            "grid_normalize", "g2p_gradu", "update_F", "bin_sort", "bin_count", "Binning",
            "BinSort", "PicError", "DUAL_DIMS", "aligned_dual", "InitUniform", "InitRandom",
            "initialize_particles", "select", "filter_particles", "halo_scatter",
-           "halo_gather", "AoSoA", "AoSoASlice", "permute", "G2P", "P2G", "Instance", "Batch",
+           "halo_gather", "AoSoA", "AoSoASlice", "permute", "G2P", "P2G", "Instance", "Batch", "FieldDep", "GridOperator",
            "lib", "LIB_PATH"]
 
 LIB_PATH = _abi.LIB_PATH
@@ -647,6 +647,59 @@ def permute(ctx, perm, pairs):
     rc = lib.pic_permute(ctx._bind_stream(), dt, np_, ctypes.c_void_p(perm.data_ptr()), n,
                          srcs, dsts, nc)
     check(rc, "pic_permute")
+
+
+@dataclass
+class FieldDep:
+    """A gather or scatter dependency of a GridOperator: a grid window, its
+    owned entities [own_lo, own_lo + own_n) and its periodic dimensions."""
+    field: GridField
+    own_lo: Sequence[int]
+    own_n: Sequence[int]
+    periodic: Sequence[bool] = (False, False, False)
+
+    def c(self):
+        i3 = ctypes.c_int32 * 3
+        return _abi.FieldDep(self.field.c(), self.field.ncomp, i3(*[int(v) for v in self.own_lo]),
+                             i3(*[int(v) for v in self.own_n]),
+                             i3(*[int(bool(v)) for v in self.periodic]))
+
+
+class GridOperator:
+    """GridOperator::apply for particle operations on one GPU
+    (Picasso_GridOperator.hpp:459-496, pic_grid_operator_apply): gather the
+    gather fields' periodic halos, zero the scatter fields, run body(ctx)
+    (the particle transfers, e.g. G2P / P2G / APIC calls), then scatter-sum
+    the scatter fields' periodic halos."""
+
+    def __init__(self, gather=(), scatter=()):
+        self.gather, self.scatter = list(gather), list(scatter)
+
+    def apply(self, ctx, body):
+        deps = self.gather + self.scatter
+        if not deps:
+            raise ValueError("GridOperator needs at least one field")
+        dt = _dtype_code(deps[0].field.tensor)
+        for d in deps:
+            _grid(d.field, "field", dt)
+        ga = (_abi.FieldDep * max(1, len(self.gather)))(*[d.c() for d in self.gather])
+        sa = (_abi.FieldDep * max(1, len(self.scatter)))(*[d.c() for d in self.scatter])
+        failure = []
+
+        def thunk(_ctx, _user):
+            try:
+                body(ctx)
+                return 0
+            except Exception as e:  # reported after the C call returns
+                failure.append(e)
+                return _abi.PIC_ERR_ARG
+
+        cb = _abi.APPLY_FN(thunk)
+        rc = lib.pic_grid_operator_apply(ctx._bind_stream(), dt, len(self.gather), ga,
+                                         len(self.scatter), sa, cb, None)
+        if failure:
+            raise failure[0]
+        check(rc, "pic_grid_operator_apply")
 
 
 @dataclass

@@ -55,6 +55,13 @@ class Instance(ctypes.Structure):
                 ("geom", Binning), ("m_i", Grid), ("mu_i", Grid)]
 
 
+class FieldDep(ctypes.Structure):
+    _fields_ = [("grid", Grid), ("ncomp", ctypes.c_int32), ("own_lo", ctypes.c_int32 * 3),
+                ("own_n", ctypes.c_int32 * 3), ("periodic", ctypes.c_int32 * 3)]
+
+
+APPLY_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
+
 _P = ctypes.POINTER
 _c_int, _c_i64, _c_u32, _c_double, _vp = (ctypes.c_int, ctypes.c_int64, ctypes.c_uint32,
                                           ctypes.c_double, ctypes.c_void_p)
@@ -112,6 +119,8 @@ PROTOTYPES = {
     "pic_batch_apic_p2g": (_c_int, [_vp, _vp, _P(PField), _P(PField), _P(PField), _c_u32]),
     "pic_batch_apic_g2p_normalized": (_c_int, [_vp, _vp, _P(PField), _P(PField)]),
     "pic_batch_apic_step": (_c_int, [_vp, _vp, _P(PField), _P(PField), _P(PField), _c_u32]),
+    "pic_grid_operator_apply": (_c_int, [_vp, _c_int, _c_int, _P(FieldDep), _c_int,
+                                         _P(FieldDep), APPLY_FN, _vp]),
     "pic_graph_begin": (_c_int, [_vp]),
     "pic_graph_end": (_c_int, [_vp, _P(_vp)]),
     "pic_graph_launch": (_c_int, [_vp, _vp]),

@@ -114,3 +114,54 @@ extern "C" int pic_grid_halo(pic_context* ctx, int op, int dtype, const pic_grid
     else go(float{});
     return launch_status();
 }
+
+// ---------------------------------------------------------------------------
+// GridOperator's apply for a particle operation on one GPU
+// (Picasso_GridOperator.hpp:459-496): gather halos, zeroed scatter fields,
+// the operation, scatter-sum halos.
+
+static bool any_periodic(const pic_field_dep& f)
+{
+    return f.periodic[0] || f.periodic[1] || f.periodic[2];
+}
+
+extern "C" int pic_grid_operator_apply(pic_context* ctx, int dtype, int ngather,
+                                       const pic_field_dep* gather, int nscatter,
+                                       const pic_field_dep* scatter, pic_apply_fn body,
+                                       void* user)
+{
+    if (!ctx || !body || ngather < 0 || nscatter < 0 || (ngather > 0 && !gather) ||
+        (nscatter > 0 && !scatter) || (dtype != PIC_F64 && dtype != PIC_F32))
+        return PIC_ERR_ARG;
+    for (int k = 0; k < ngather + nscatter; ++k) {
+        const pic_field_dep& f = k < ngather ? gather[k] : scatter[k - ngather];
+        if (!valid_grid(&f.grid) || f.ncomp < 1) return PIC_ERR_ARG;
+    }
+    // Gather distributed dependencies (:465-466): ghosts take their images.
+    for (int k = 0; k < ngather; ++k) {
+        const pic_field_dep& f = gather[k];
+        if (!any_periodic(f)) continue;
+        if (const int rc = pic_grid_halo(ctx, PIC_HALO_GATHER, dtype, &f.grid, f.ncomp, f.own_lo,
+                                         f.own_n, f.periodic))
+            return rc;
+    }
+    // Scatter fields start from zero (createDependencies, :532-533).
+    {
+        DeviceGuard dg(ctx->device);
+        if (!dg.ok) return PIC_ERR_HIP;
+        for (int k = 0; k < nscatter; ++k)
+            if (const int rc = zero_grid(ctx, dtype, scatter[k].grid, scatter[k].ncomp)) return rc;
+    }
+    // The operation (applyOp, :486-487): the caller's transfers.
+    if (const int rc = body(ctx, user)) return rc;
+    // Scatter distributed dependencies (:489-491): ghosts add into their
+    // images.
+    for (int k = 0; k < nscatter; ++k) {
+        const pic_field_dep& f = scatter[k];
+        if (!any_periodic(f)) continue;
+        if (const int rc = pic_grid_halo(ctx, PIC_HALO_SCATTER_SUM, dtype, &f.grid, f.ncomp,
+                                         f.own_lo, f.own_n, f.periodic))
+            return rc;
+    }
+    return PIC_OK;
+}

@@ -274,6 +274,76 @@ using PolyStage = BrickStage<(PolyBrickShape<ORDER, T>::CAP + PolyBrickShape<ORD
                                  PolyBrickShape<ORDER, T>::NT,
                              PolyBrickShape<ORDER, T>::NT, PolyBrickShape<ORDER, T>::BINS>;
 
+// Stage chunk [c0, c1) of a brick's sorted slots for the PolyPIC p2g
+// bodies: per particle (f, m) and the rows of A^-1 (A = I + dt U, inverted
+// once here) in the stage_at groups, its index in pidx, its bin's
+// chunk-local range in the returned range buffer; a particle whose knot
+// base differs from its bin is pushed to the stale list (f0 = STALE_F).
+template <int ORDER, typename T>
+__device__ __forceinline__ uint32_t* polypic_stage_chunk(const PolyBrickArgs<T>& g,
+                                                         PolyStage<ORDER, T>& st, uint32_t beg,
+                                                         uint32_t c0, uint32_t c1, uint32_t kb,
+                                                         const int f0[3], T* lds,
+                                                         uint32_t* pidx)
+{
+    using S = PolyBrickShape<ORDER, T>;
+    using T4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
+    constexpr int PB = S::PB, BINS = S::BINS, NT = S::NT, CAP = S::CAP;
+    constexpr int FIRST = Knots<ORDER>::FIRST;
+    constexpr int K = (CAP + NT - 1) / NT;
+    const PolyArgs<T>& a = g.a;
+    const T dt = (T)a.dt;
+    uint32_t* rng = st.begin_chunk(g, a.perm, beg, c0, c1);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t s = c0 + threadIdx.x + uint32_t(k * NT);
+        if (s >= c1) continue;
+        const int64_t p = int64_t(st.cs.perm[k]);
+        const int q = int(s - c0);
+        const int lb = int(st.cs.key[k] - kb);
+        const int eb[3] = {f0[0] + lb / (PB * PB), f0[1] + (lb / PB) % PB, f0[2] + lb % PB};
+        const int64_t xb = a.x.base(p);
+        T4 v;
+        T* fr = &v.x;
+        bool ok = true;
+#pragma unroll
+        for (int dim = 0; dim < 3; ++dim) {
+            double fd;
+            const double kb0 = knot_base<ORDER>(double(a.x.at(xb, dim)), a.lat.o[dim],
+                                                 a.lat.rdx, fd);
+            ok = ok && int(kb0) + FIRST == eb[dim];
+            fr[dim] = T(fd);
+        }
+        if (!ok) {
+            push_stale(g, s);
+            v.x = T(STALE_F);
+        }
+        v.w = a.m.at(a.m.base(p), 0);
+        *reinterpret_cast<T4*>(lds + stage_at(q, 0)) = v;
+        // A = I + dt U, U(r, d) = u_p(1 + r, d) (:81-84); rows of A^-1.
+        const int64_t ub = a.u.base(p);
+        T am[9], ai[9];
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int e = 0; e < 3; ++e)
+                am[r * 3 + e] = (r == e ? T(1) : T(0)) + dt * a.u.at(ub, (1 + r) * 3 + e);
+        inverse3(am, det3(am), ai);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+            v.x = ai[r * 3 + 0];
+            v.y = ai[r * 3 + 1];
+            v.z = ai[r * 3 + 2];
+            v.w = T(0);
+            *reinterpret_cast<T4*>(lds + stage_at(q, 1 + r)) = v;
+        }
+        pidx[q] = uint32_t(p);
+        st.cs.template note<BINS>(k, q, kb, rng);
+    }
+    st.end_chunk();
+    return rng;
+}
+
 template <int ORDER, typename T, bool STAG>
 __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
                                                       const BrickItem& it,
@@ -286,8 +356,6 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
     constexpr int PB = S::PB, NK = S::NK, NN = S::NN, BINS = S::BINS, NT = S::NT, CAP = S::CAP;
     constexpr int NM = Modes<ORDER>::NM;
     constexpr int E = ORDER == 1 ? 2 : 3;
-    constexpr int FIRST = Knots<ORDER>::FIRST;
-    constexpr int K = (CAP + NT - 1) / NT;
     const PolyArgs<T>& a = g.a;
     const uint32_t beg = it.beg, end = it.end;
     int f0[3];
@@ -296,60 +364,12 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
 
     const int l = int(threadIdx.x) >> 2, comp = int(threadIdx.x) & 3;
     const int d = comp > 0 ? comp - 1 : 0;
-    const T dt = (T)a.dt;
     T acc[NN];
 #pragma unroll
     for (int n = 0; n < NN; ++n) acc[n] = T(0);
     for (uint32_t c0 = beg; c0 < end; c0 += CAP) {
         const uint32_t c1 = min(end, c0 + uint32_t(CAP));
-        uint32_t* rng = st.begin_chunk(g, a.perm, beg, c0, c1);
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint32_t s = c0 + threadIdx.x + uint32_t(k * NT);
-            if (s >= c1) continue;
-            const int64_t p = int64_t(st.cs.perm[k]);
-            const int q = int(s - c0);
-            const int lb = int(st.cs.key[k] - kb);
-            const int eb[3] = {f0[0] + lb / (PB * PB), f0[1] + (lb / PB) % PB, f0[2] + lb % PB};
-            const int64_t xb = a.x.base(p);
-            T4 v;
-            T* fr = &v.x;
-            bool ok = true;
-#pragma unroll
-            for (int dim = 0; dim < 3; ++dim) {
-                double fd;
-                const double kb0 = knot_base<ORDER>(double(a.x.at(xb, dim)), a.lat.o[dim],
-                                                     a.lat.rdx, fd);
-                ok = ok && int(kb0) + FIRST == eb[dim];
-                fr[dim] = T(fd);
-            }
-            if (!ok) {
-                push_stale(g, s);
-                v.x = T(STALE_F);
-            }
-            v.w = a.m.at(a.m.base(p), 0);
-            *reinterpret_cast<T4*>(lds + stage_at(q, 0)) = v;
-            // A = I + dt U, U(r, d) = u_p(1 + r, d) (:81-84); rows of A^-1.
-            const int64_t ub = a.u.base(p);
-            T am[9], ai[9];
-#pragma unroll
-            for (int r = 0; r < 3; ++r)
-#pragma unroll
-                for (int e = 0; e < 3; ++e)
-                    am[r * 3 + e] = (r == e ? T(1) : T(0)) + dt * a.u.at(ub, (1 + r) * 3 + e);
-            inverse3(am, det3(am), ai);
-#pragma unroll
-            for (int r = 0; r < 3; ++r) {
-                v.x = ai[r * 3 + 0];
-                v.y = ai[r * 3 + 1];
-                v.z = ai[r * 3 + 2];
-                v.w = T(0);
-                *reinterpret_cast<T4*>(lds + stage_at(q, 1 + r)) = v;
-            }
-            pidx[q] = uint32_t(p);
-            st.cs.template note<BINS>(k, q, kb, rng);
-        }
-        st.end_chunk();
+        uint32_t* rng = polypic_stage_chunk<ORDER, T>(g, st, beg, c0, c1, kb, f0, lds, pidx);
         __syncthreads();
         if (c0 == beg) w.fetch();
         const uint32_t q0 = rng[l], q1 = rng[BINS + l];

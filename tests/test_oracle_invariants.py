@@ -431,3 +431,78 @@ def test_polypic_affine_roundtrip(order):
     u = orpic.grid_normalize(gm, gmu)
     t = gm > 0
     np.testing.assert_allclose(u[t], gu[t], atol=1e-9)
+
+
+_EXPS27 = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1),
+           (1, 1, 1)] + [e for e in itertools.product(range(3), repeat=3) if max(e) == 2]
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_polypic27_quadratic_round_trip(order):
+    """Pins the quadratic modes of p2g against those of g2p: the spline
+    interpolant of u = x_a x_b is x_a x_b + delta_ab D (D = h^2/4 for
+    quadratic, h^2/3 for cubic B-splines, the constant second moment), a
+    quadratic, so its 27 Taylor modes at x_p represent it exactly; p2g at
+    dt = 0 evaluates them at the nodes, and normalisation returns
+    x_a x_b + delta_ab D at every node with mass.  A p2g whose mode
+    convention differed from g2p's (exponent order, the 1/2 of the second
+    derivative) would not."""
+    rng, x, lo, n = _poly_setup(order, 61 + order)
+    X = _node_coords(lo, n)
+    field = np.stack([X[..., 0] * X[..., 1], X[..., 2] * X[..., 2], X[..., 0] * X[..., 2]], -1)
+    D = H * H / (4.0 if order == 2 else 3.0)
+    modes = orpic.polypic_g2p(order, NODE, LOW, H, lo, n, x, field, 3)
+    m = rng.uniform(0.5, 1.0, len(x))
+    gm = np.zeros(tuple(n))
+    gmu = np.zeros(tuple(n) + (3,))
+    orpic.polypic_p2g(order, NODE, LOW, H, lo, n, x, m, modes, modes, 0.0, gmu, gm)
+    u = orpic.grid_normalize(gm, gmu)
+    t = gm > 1e-6 * gm.max()
+    want = field + np.array([0.0, D, 0.0])
+    np.testing.assert_allclose(u[t], want[t], atol=1e-11)
+
+
+@pytest.mark.parametrize("kind", ["diagonal", "nilpotent"])
+def test_polypic27_backtrack_known_answer(kind):
+    """p2g with dt != 0: y = (I + dt U)^-1 d with U(a, d) = u_p(1 + a, d).
+    For a diagonal U, y_a = d_a / (1 + dt u_aa); for U with one entry s at
+    (0, 1), (I + dt U)^-1 = I - dt U and y_0 = d_0 - dt s d_1 (fixing the
+    orientation of U).  Field modes c_0, the y_0 and y_1 modes and the y_0^2
+    mode; the node momentum is sum_p m_p w_n (c_0 + g_0 y_0 + g_1 y_1 +
+    q y_0^2) with w_n and d_n from the spline evaluation."""
+    order, dt = 3, 0.05
+    rng, x, lo, n = _poly_setup(order, 71)
+    P = len(x)
+    nm = orpic.polypic_nmode(order)
+    m = rng.uniform(0.5, 1.0, P)
+    c = np.zeros((P, nm, 1))
+    c0, g0, g1, q = rng.normal(size=(4, P))
+    c[:, 0, 0], c[:, 1, 0], c[:, 2, 0] = c0, g0, g1
+    c[:, _EXPS27.index((2, 0, 0)), 0] = q
+    u = np.zeros((P, nm, 3))
+    diag = rng.uniform(-2.0, 2.0, size=(P, 3))
+    s = rng.uniform(-3.0, 3.0, P)
+    if kind == "diagonal":
+        for a in range(3):
+            u[:, 1 + a, a] = diag[:, a]
+    else:
+        u[:, 1, 1] = s
+    gm = np.zeros(tuple(n))
+    gmu = np.zeros(tuple(n) + (1,))
+    orpic.polypic_p2g(order, NODE, LOW, H, lo, n, x, m, u, c, dt, gmu, gm)
+    want = np.zeros(tuple(n))
+    for p in range(P):
+        sp = orpic.spline(order, NODE, LOW, H, x[p])
+        for i in range(4):
+            for j in range(4):
+                for k in range(4):
+                    d = np.array([sp["d"][0][i], sp["d"][1][j], sp["d"][2][k]])
+                    if kind == "diagonal":
+                        y = d / (1.0 + dt * diag[p])
+                    else:
+                        y = np.array([d[0] - dt * s[p] * d[1], d[1], d[2]])
+                    wn = sp["w"][0][i] * sp["w"][1][j] * sp["w"][2][k]
+                    node = (sp["s"][0][i] - lo[0], sp["s"][1][j] - lo[1], sp["s"][2][k] - lo[2])
+                    want[node] += m[p] * wn * (c0[p] + g0[p] * y[0] + g1[p] * y[1]
+                                               + q[p] * y[0] * y[0])
+    np.testing.assert_allclose(gmu[..., 0], want, atol=1e-12 * np.abs(want).max())
