@@ -85,7 +85,10 @@ class Step:
     def reorder(self, check=True):
         """Store the particles in bin order (what a PIC code does when it
         re-sorts its particle arrays: sort, then pic_permute into a second
-        set of arrays); later sorts still run in full."""
+        set of arrays); later sorts still run in full.  With check=False
+        only the device work is done (the timed form): no synchronisation,
+        and the bench's own bookkeeping (the reset copy of c_p, the input
+        order of each slot) is left stale."""
         self.pa.bin_sort(self.ctx, self.mesh, self.x, self.geom, out=self.bins)
         if check:
             self.ctx.synchronize()  # raises if a particle fell outside the bins
@@ -94,8 +97,9 @@ class Step:
         self.pa.permute(self.ctx, self.bins.perm, [(getattr(self, k), new[k]) for k in new])
         for k, v in new.items():
             setattr(self, k, v)
-        self.c0 = self.c.clone()
-        self.order = self.order[self.bins.perm.long().cpu().numpy()]
+        if check:
+            self.c0 = self.c.clone()
+            self.order = self.order[self.bins.perm.long().cpu().numpy()]
 
     def outputs(self):
         """Grid mass, momentum and c_p in input order, as float64 numpy."""
