@@ -77,7 +77,21 @@ typedef enum pic_entity {
 
 typedef enum pic_dtype { PIC_F64 = 0, PIC_F32 = 1 } pic_dtype;
 
-enum { PIC_P2G_ZERO_FIRST = 1u };
+/* p2g flags.  PIC_P2G_ZERO_FIRST zeroes the grid windows first (as
+ * GridOperator resets its scatter fields, Picasso_GridOperator.hpp:532-533).
+ * PIC_P2G_DETERMINISTIC (pic_apic_p2g with bins the brick kernels take; no
+ * reference counterpart: Kokkos' ScatterView sums in any order) makes the
+ * result bit-identical from run to run for the same inputs and bins: the
+ * bricks are flushed in eight passes, one per brick colour (the parities of
+ * the brick coordinates), so no two bricks in flight share a node and every
+ * node receives its per-brick sums in colour order without atomics; the
+ * particles whose bins are stale are then applied one after another in slot
+ * order.  Slower than the default (eight launches; stale particles
+ * serially); without bins, or with bins the brick kernels cannot use, the
+ * call returns PIC_ERR_UNSUPPORTED before doing anything.  More than 8192
+ * stale particles are still applied, but in an unspecified order, and
+ * pic_synchronize reports PIC_ERR_CAPACITY. */
+enum { PIC_P2G_ZERO_FIRST = 1u, PIC_P2G_DETERMINISTIC = 2u };
 
 /* Uniform local mesh (Picasso_UniformMesh.hpp:84-190): `low` is the physical
  * position of local node (0,0,0), ghost layers included; `h` the isotropic
@@ -154,7 +168,10 @@ int pic_context_create(pic_context** ctx, int device);
 int pic_context_destroy(pic_context* ctx);
 /* hipStream_t passed as void*; NULL selects the null stream. */
 int pic_set_stream(pic_context* ctx, void* stream);
-/* Wait for the stream; return the first latched device error and clear it. */
+/* Wait for the stream; return the latched device errors and clear them:
+ * PIC_ERR_BOUNDS when a stencil or particle left its window or bins,
+ * otherwise PIC_ERR_CAPACITY when a deterministic p2g could not order its
+ * stale particles (see PIC_P2G_DETERMINISTIC). */
 int pic_synchronize(pic_context* ctx);
 const char* pic_status_string(int status);
 int pic_api_version(void);
@@ -164,7 +181,8 @@ int pic_api_version(void);
  * Reference signature p2g(m_p, c_p, m_i, mu_i, sd): x and the mesh stand in
  * for the spline data sd (createSpline, Picasso_ParticleInterpolation.hpp:
  * 88-105).  c_p has 12 components (4x3).  mu_i has 3 components for
- * Node/Cell, 1 for Face/Edge.  bins may be NULL. */
+ * Node/Cell, 1 for Face/Edge.  bins may be NULL.  flags: PIC_P2G_ZERO_FIRST,
+ * PIC_P2G_DETERMINISTIC. */
 int pic_apic_p2g(pic_context* ctx, int order, int entity, int dtype,
                  const pic_mesh* mesh, int64_t np, const pic_pfield* x,
                  const pic_pfield* m_p, const pic_pfield* c_p,

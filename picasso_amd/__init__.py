@@ -366,9 +366,11 @@ def _bins_ptr(bins, np_):
 
 class APIC:
     @staticmethod
-    def p2g(ctx, m_p, c_p, m_i, mu_i, sd, zero_first=False):
+    def p2g(ctx, m_p, c_p, m_i, mu_i, sd, zero_first=False, deterministic=False):
         """APIC::p2g(m_p, c_p, m_i, mu_i, sd) (Picasso_APIC.hpp:55-402).
-        c_p is [np, 12] (row-major 4x3); accumulates into m_i, mu_i."""
+        c_p is [np, 12] (row-major 4x3); accumulates into m_i, mu_i.
+        deterministic: bit-identical sums from run to run (needs bins the
+        brick kernels take; PIC_P2G_DETERMINISTIC)."""
         x = sd.x
         np_, dt = x.shape[0], _dtype_code(x)
         args = (ctypes.byref(sd.mesh.c()), np_, ctypes.byref(_pfield(x, 3, "x")),
@@ -376,7 +378,9 @@ class APIC:
                 ctypes.byref(_pfield(c_p, 12, "c_p", np_, dt)),
                 ctypes.byref(_grid(m_i, "m_i", dt, 1)),
                 ctypes.byref(_grid(mu_i, "mu_i", dt, _ncomp_grid(sd.location, 3))),
-                _bins_ptr(sd.bins, np_), _abi.P2G_ZERO_FIRST if zero_first else 0)
+                _bins_ptr(sd.bins, np_),
+                (_abi.P2G_ZERO_FIRST if zero_first else 0) |
+                (_abi.P2G_DETERMINISTIC if deterministic else 0))
         check(lib.pic_apic_p2g(ctx._bind_stream(), sd.order, sd.location, dt, *args),
               "pic_apic_p2g")
 

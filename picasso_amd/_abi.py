@@ -22,6 +22,7 @@ PIC_ERR_CAPACITY = -6
 NODE, CELL, FACE_I, FACE_J, FACE_K, EDGE_I, EDGE_J, EDGE_K = range(8)
 F64, F32 = 0, 1
 P2G_ZERO_FIRST = 1
+P2G_DETERMINISTIC = 2
 
 
 class Mesh(ctypes.Structure):

@@ -18,8 +18,12 @@ template <int ORDER, bool STAG, typename T>
 __global__ __launch_bounds__(256) void k_apic_p2g_scatter(ApicArgs<T> a)
 {
     const int64_t n = a.list ? int64_t(a.list[0]) : a.np;
-    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
-         i += int64_t(gridDim.x) * blockDim.x) {
+    // A short list in deterministic mode: work-item 0 alone, in list order.
+    const bool serial = a.list && n <= int64_t(a.serial_max);
+    if (serial && (blockIdx.x | threadIdx.x) != 0) return;
+    const int64_t stride = serial ? 1 : int64_t(gridDim.x) * blockDim.x;
+    for (int64_t i = serial ? 0 : int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+         i += stride) {
         const int64_t s = a.list ? int64_t(a.list[1 + i]) : i;
         const int64_t p = a.perm ? int64_t(a.perm[s]) : s;
         ApicParticle<T> pt;
@@ -125,7 +129,7 @@ struct G2PGatherLaunch {
 int apic_p2g_list(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                   int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
                   const pic_grid& gm, const pic_grid& gmu, const uint32_t* perm,
-                  const uint32_t* list)
+                  const uint32_t* list, bool serial)
 {
     const bool stag = stag_dim(entity) >= 0;
     TimedLaunch tl(ctx, "apic_p2g_stale");
@@ -134,12 +138,14 @@ int apic_p2g_list(pic_context* ctx, int order, int entity, int dtype, const pic_
                                                      perm),
                                    ctx->stream};
         L.a.list = list;
+        L.a.serial_max = serial ? STALE_SORT_MAX : 0u;
         dispatch_order_stag<double>(order, stag, L);
     } else {
         P2GScatterLaunch<float> L{make_args<float>(ctx, entity, mesh, np, &x, &m, &c, &gm, gmu,
                                                    perm),
                                   ctx->stream};
         L.a.list = list;
+        L.a.serial_max = serial ? STALE_SORT_MAX : 0u;
         dispatch_order_stag<float>(order, stag, L);
     }
     return launch_status();
@@ -197,6 +203,8 @@ extern "C" int pic_apic_p2g(pic_context* ctx, int order, int entity, int dtype,
     if (np > 0 && (!valid_pfield(x) || !valid_pfield(m_p) || !valid_pfield(c_p)))
         return PIC_ERR_ARG;
     if (bins && !bins->perm) return PIC_ERR_ARG;
+    const bool det = (flags & PIC_P2G_DETERMINISTIC) != 0;
+    if (det && !(bins && apic_brick_supported(order, entity, *bins))) return PIC_ERR_UNSUPPORTED;
     DeviceGuard g(ctx->device);
     if (!g.ok) return PIC_ERR_HIP;
     const bool stag = stag_dim(entity) >= 0;
@@ -209,9 +217,11 @@ extern "C" int pic_apic_p2g(pic_context* ctx, int order, int entity, int dtype,
     if (np == 0) return PIC_OK;
     if (bins) {
         int rc = apic_p2g_brick(ctx, order, entity, dtype, *mesh, np, *x, *m_p, *c_p,
-                                *m_i, *mu_i, *bins);
+                                *m_i, *mu_i, *bins, det);
         if (rc != PIC_ERR_UNSUPPORTED) return rc;
     }
+    // The per-particle kernel's atomics land in any order.
+    if (det) return PIC_ERR_UNSUPPORTED;
     TimedLaunch tl(ctx, "apic_p2g");
     const uint32_t* perm = bins ? bins->perm : nullptr;
     if (dtype == PIC_F64) {

@@ -14,6 +14,9 @@ struct ApicArgs {
     int64_t np;
     const uint32_t* perm;  // sorted slot -> particle, or null (identity)
     const uint32_t* list;  // [count, slots...] to process instead of 0..np-1, or null
+    // p2g over a list: when the count is at most serial_max (non-zero), one
+    // work-item applies the particles one after another in list order.
+    uint32_t serial_max;
     PField<T> x, m, c;     // positions (3), mass (1), c_p (12)
     GView<T> gm, gmu;      // grid mass, grid momentum (or velocity for g2p)
     GView<T> gnm;          // g2p with fused normalisation: grid mass (else null data)
@@ -264,6 +267,7 @@ inline ApicArgs<T> make_args(pic_context* ctx, int entity, const pic_mesh& mesh,
     a.np = np;
     a.perm = perm;
     a.list = nullptr;
+    a.serial_max = 0u;
     a.x = make_pfield<T>(*x);
     a.m = m ? make_pfield<T>(*m) : PField<T>{};
     a.c = make_pfield<T>(*c);
@@ -297,21 +301,30 @@ inline void dispatch_order_stag(int order, bool stag, Launch& L)
     }
 }
 
-// Scatter/gather over a device-side slot list (pic_apic.hip).
+// Longest stale list a deterministic p2g orders (k_sort_stale) and applies
+// one particle after another.
+constexpr uint32_t STALE_SORT_MAX = 8192;
+
+// Scatter/gather over a device-side slot list (pic_apic.hip).  With
+// `serial`, a list of at most STALE_SORT_MAX particles is applied by one
+// work-item in list order (longer ones by the whole grid).
 int apic_p2g_list(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                   int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
                   const pic_grid& gm, const pic_grid& gmu, const uint32_t* perm,
-                  const uint32_t* list);
+                  const uint32_t* list, bool serial = false);
 int apic_g2p_list(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                   int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
                   const pic_pfield& c, const uint32_t* perm, const uint32_t* list,
                   const FUpdate* fu);
 
 // Brick scheme (pic_apic_brick.hip).  Return PIC_ERR_UNSUPPORTED to fall
-// back to the scatter/gather kernels.
+// back to the scatter/gather kernels.  `det`: the deterministic p2g (colour
+// passes, stale particles one after another in slot order).
 int apic_p2g_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                    int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
-                   const pic_grid& gm, const pic_grid& gmu, const pic_bins& bins);
+                   const pic_grid& gm, const pic_grid& gmu, const pic_bins& bins, bool det);
+// Whether the brick kernels take (order, entity) on these bins.
+bool apic_brick_supported(int order, int entity, const pic_bins& bins);
 int apic_g2p_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                    int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
                    const pic_pfield& c, const pic_bins& bins, const FUpdate* fu);

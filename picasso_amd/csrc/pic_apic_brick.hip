@@ -19,6 +19,63 @@ __global__ __launch_bounds__((P2GShape<ORDER, T>::NT), (P2GShape<ORDER, T>::WAVE
     });
 }
 
+// Deterministic p2g, one launch per brick colour: the bricks of colour c
+// (grouped by k_colour_items into `items`, counts and offsets in `colour`)
+// under a static walk, flushed with plain read-modify-writes.  No two
+// bricks of a colour share a node, so every node receives its partial sums
+// in colour order whatever the scheduling.
+template <int ORDER, typename T, bool STAG>
+__global__ __launch_bounds__((P2GShape<ORDER, T>::NT), (P2GShape<ORDER, T>::WAVES)) void
+k_apic_p2g_brick_colour(BrickArgs<T> g, const BrickItem* items, const uint32_t* colour, uint32_t c)
+{
+    using S = P2GShape<ORDER, T>;
+    __shared__ __align__(16) T lds[S::LDS];
+    __shared__ uint32_t rng[2 * 2 * S::BINS];
+    __shared__ BrickItem q[2];
+    for (int i = threadIdx.x; i < 4 * S::BINS; i += S::NT) rng[i] = 0u;
+    BrickArgs<T> gc = g;
+    gc.items = items + colour[8 + c];
+    gc.active = colour + c;  // active[0]: this colour's brick count
+    P2GState<ORDER, T> st{rng, 0, false, {}};
+    walk_bricks<false>(gc, q, [&](const BrickItem& it, BrickWalk<false, BrickArgs<T>>& w) {
+        p2g_one_brick<ORDER, T, STAG, true, false>(gc, it, w, lds, st);
+    });
+}
+
+// Stale list [count, slots...] sorted ascending in place (one workgroup,
+// bitonic in LDS), so that applying its particles one after another gives
+// the same sums on every run.  Longer lists than STALE_SORT_MAX stay as
+// they are and raise ERRBIT_CAPACITY.
+__global__ __launch_bounds__(1024) void k_sort_stale(uint32_t* list, int* err)
+{
+    __shared__ uint32_t s[STALE_SORT_MAX];
+    const uint32_t n = list[0];
+    if (n <= 1u) return;
+    if (n > STALE_SORT_MAX) {
+        if (threadIdx.x == 0) atomicOr(err, ERRBIT_CAPACITY);
+        return;
+    }
+    uint32_t m = 2;
+    while (m < n) m <<= 1;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) s[i] = i < n ? list[1 + i] : ~0u;
+    __syncthreads();
+    for (uint32_t k = 2; k <= m; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+                const uint32_t ij = i ^ j;
+                if (ij > i) {
+                    const uint32_t a = s[i], b = s[ij];
+                    if ((a > b) == ((i & k) == 0)) {
+                        s[i] = b;
+                        s[ij] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) list[1 + i] = s[i];
+}
+
 template <int BB, int ORDER, typename T, bool GF, bool STAG>
 __global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
 {
@@ -34,9 +91,10 @@ template <typename T>
 static int make_brick_args(pic_context* ctx, int order, int entity, const pic_mesh& mesh,
                             int64_t np, const pic_pfield& x, const pic_pfield* m,
                             const pic_pfield& c, const pic_grid* gm, const pic_grid& gmu,
-                            const pic_grid* gmass, const pic_bins& bins, BrickArgs<T>& g)
+                            const pic_grid* gmass, const pic_bins& bins, BrickArgs<T>& g,
+                            size_t extra_bytes = 0, void** extra = nullptr)
 {
-    const int rc = setup_walk(ctx, order, entity, np, bins, g);
+    const int rc = setup_walk(ctx, order, entity, np, bins, g, extra_bytes, extra);
     if (rc) return rc;
     g.a = make_args<T>(ctx, entity, mesh, np, &x, m, &c, gm, gmu, bins.perm);
     if (gmass) g.a.gnm = make_gview<T>(*gmass);
@@ -48,10 +106,20 @@ struct P2GBrickLaunch {
     pic_context* ctx;
     const BrickArgs<T>& g;
     bool stag;
+    // Deterministic: the colour passes over `items` / `colour`.
+    const BrickItem* items = nullptr;
+    const uint32_t* colour = nullptr;
     template <int O, bool STAG>
     void launch()
     {
         constexpr int NT = P2GShape<O, T>::NT;
+        if (items) {
+            const unsigned nbk =
+                persistent_blocks(ctx, k_apic_p2g_brick_colour<O, T, STAG>, NT, 0, g.nbricks);
+            for (uint32_t c = 0; c < 8; ++c)
+                k_apic_p2g_brick_colour<O, T, STAG><<<nbk, NT, 0, ctx->stream>>>(g, items, colour, c);
+            return;
+        }
         const unsigned nbk =
             persistent_blocks(ctx, k_apic_p2g_brick<O, T, STAG>, NT, 0, g.nbricks);
         k_apic_p2g_brick<O, T, STAG><<<nbk, NT, 0, ctx->stream>>>(g);
@@ -103,19 +171,38 @@ template <typename T>
 static int p2g_brick_run(pic_context* ctx, int order, int entity, const pic_mesh& mesh, int64_t np,
                          const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
                          const pic_grid& gm, const pic_grid& gmu, const pic_bins& bins,
-                         int dtype)
+                         int dtype, bool det)
 {
     if (brick_edge(bins.geom) != PB) return PIC_ERR_UNSUPPORTED;
     BrickArgs<T> g;
+    // Deterministic: 16 words of colour counts and offsets, then the walk
+    // items grouped by colour.
+    const int64_t nb = pic_bin_count(&bins.geom);
+    void* extra = nullptr;
+    const size_t extra_bytes = det && nb > 0 ? 64 + size_t(nb) * sizeof(BrickItem) : 0;
     if (const int rc = make_brick_args<T>(ctx, order, entity, mesh, np, x, &m, c, &gm, gmu, nullptr,
-                                          bins, g))
+                                          bins, g, extra_bytes, &extra))
         return rc;
-    {
-        TimedLaunch tl(ctx, "apic_p2g");
-        P2GBrickLaunch<T>{ctx, g, stag_dim(entity) >= 0}.by_order(order);
+    P2GBrickLaunch<T> L{ctx, g, stag_dim(entity) >= 0};
+    if (det) {
+        L.colour = static_cast<uint32_t*>(extra);
+        L.items = reinterpret_cast<const BrickItem*>(static_cast<char*>(extra) + 64);
+        k_colour_items<<<1, 1024, 0, ctx->stream>>>(g, const_cast<BrickItem*>(L.items),
+                                                     const_cast<uint32_t*>(L.colour));
         if (launch_status()) return PIC_ERR_HIP;
     }
-    return apic_p2g_list(ctx, order, entity, dtype, mesh, np, x, m, c, gm, gmu, bins.perm, g.stale);
+    {
+        TimedLaunch tl(ctx, "apic_p2g");
+        L.by_order(order);
+        if (launch_status()) return PIC_ERR_HIP;
+    }
+    if (det) {
+        // Stale particles one after another in slot order.
+        k_sort_stale<<<1, 1024, 0, ctx->stream>>>(g.stale, ctx->d_err);
+        if (launch_status()) return PIC_ERR_HIP;
+    }
+    return apic_p2g_list(ctx, order, entity, dtype, mesh, np, x, m, c, gm, gmu, bins.perm, g.stale,
+                         det);
 }
 
 template <typename T>
@@ -140,13 +227,21 @@ static int g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
                          fu);
 }
 
+bool apic_brick_supported(int order, int entity, const pic_bins& bins)
+{
+    int32_t knot_off[3];
+    return bins.perm && bins.keys && bins.bin_start && brick_edge(bins.geom) == PB &&
+           pic_bin_count(&bins.geom) > 0 && aligned_knot_offset(order, entity, bins.geom.dual, knot_off);
+}
+
 int apic_p2g_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                    int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
-                   const pic_grid& gm, const pic_grid& gmu, const pic_bins& bins)
+                   const pic_grid& gm, const pic_grid& gmu, const pic_bins& bins, bool det)
 {
     if (dtype == PIC_F32)
-        return p2g_brick_run<float>(ctx, order, entity, mesh, np, x, m, c, gm, gmu, bins, dtype);
-    return p2g_brick_run<double>(ctx, order, entity, mesh, np, x, m, c, gm, gmu, bins, dtype);
+        return p2g_brick_run<float>(ctx, order, entity, mesh, np, x, m, c, gm, gmu, bins, dtype,
+                                    det);
+    return p2g_brick_run<double>(ctx, order, entity, mesh, np, x, m, c, gm, gmu, bins, dtype, det);
 }
 
 int apic_g2p_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,

@@ -210,9 +210,11 @@ using P2GState = BrickStage<(P2GShape<ORDER, T>::CAP + P2GShape<ORDER, T>::NT - 
                                 P2GShape<ORDER, T>::NT,
                             P2GShape<ORDER, T>::NT, P2GShape<ORDER, T>::BINS>;
 
-template <int ORDER, typename T, bool STAG>
+// PLAIN flushes with plain read-modify-writes (walks over bricks with
+// disjoint footprints); DYN is the walk's kind.
+template <int ORDER, typename T, bool STAG, bool PLAIN = false, bool DYN = true>
 __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const BrickItem& it,
-                                              BrickWalk<true, BrickArgs<T>>& w, T* lds,
+                                              BrickWalk<DYN, BrickArgs<T>>& w, T* lds,
                                               P2GState<ORDER, T>& st)
 {
     using S = P2GShape<ORDER, T>;
@@ -454,7 +456,8 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
         if (r == 0) w.publish();
         __syncthreads();
         if (r == 0) st.prefetch_next(g, a.perm, w, CAP);
-        gather_flush<NK, PB, NT, S::CPR>(lds, f0, r_lo, 1, a.gm, a.gmu, a.err, STAG ? 2 : 4);
+        gather_flush<NK, PB, NT, S::CPR, T, PLAIN>(lds, f0, r_lo, 1, a.gm, a.gmu, a.err,
+                                                   STAG ? 2 : 4);
         if (r + 1 < S::NROUND) __syncthreads();
     }
 }

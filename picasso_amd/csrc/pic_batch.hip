@@ -565,6 +565,7 @@ extern "C" int pic_batch_apic_p2g(pic_context* ctx, pic_batch* b, const pic_pfie
 {
     if (!ctx || !b || b->device != ctx->device || !valid_step_fields(b, x, m_p, c_p, true))
         return PIC_ERR_ARG;
+    if (flags & PIC_P2G_DETERMINISTIC) return PIC_ERR_UNSUPPORTED;  // single instances only
     DeviceGuard dg(ctx->device);
     if (!dg.ok) return PIC_ERR_HIP;
     if ((flags & PIC_P2G_ZERO_FIRST) && b->nchunks > 0) {

@@ -292,7 +292,9 @@ __device__ __forceinline__ int stage_at(int q, int g)
 // rate), and applies one atomic per component: component r_lo + cc
 // goes to gm (cc index 0 of round 0, when gm is not null) or to component
 // (r_lo + cc - c_mom) of gmu; components from nwrite on are not written.
-template <int NK, int PB, int NT, int CPR, typename T>
+// PLAIN replaces the atomics by plain read-modify-writes, for walks in
+// which no other brick touches the footprint at the same time.
+template <int NK, int PB, int NT, int CPR, typename T, bool PLAIN = false>
 __device__ __forceinline__ void gather_flush(const T* sums, const int f0[3], int r_lo, int c_mom,
                                              const GView<T>& gm, const GView<T>& gmu, int* err,
                                              int nwrite = 1 << 30)
@@ -333,9 +335,56 @@ __device__ __forceinline__ void gather_flush(const T* sums, const int f0[3], int
             const int c = r_lo + cc;
             const T v = T(tot[cc]);
             if (v == T(0) || c >= nwrite) continue;
-            if (c < c_mom) atomic_add(gm.data + gm.off(ni, nj, nk), v);
-            else atomic_add(gmu.data + off + (c - c_mom) * gmu.cs, v);
+            T* dst = c < c_mom ? gm.data + gm.off(ni, nj, nk) : gmu.data + off + (c - c_mom) * gmu.cs;
+            if constexpr (PLAIN) *dst = *dst + v;
+            else atomic_add(dst, v);
         }
+    }
+}
+
+// Colour of brick b (0..7): the parities of its brick coordinates.  Two
+// bricks of one colour are at least 2 bricks apart in some dimension, so
+// their node footprints (at most 2 PB nodes wide) never overlap, and a walk
+// over one colour's bricks can flush its nodes without atomics.
+template <typename G>
+__device__ __forceinline__ uint32_t brick_colour(const G& g, uint32_t b)
+{
+    b -= g.brick_base;
+    const uint32_t bk = b % uint32_t(g.nb[2]);
+    const uint32_t bj = (b / uint32_t(g.nb[2])) % uint32_t(g.nb[1]);
+    const uint32_t bi = b / (uint32_t(g.nb[2]) * uint32_t(g.nb[1]));
+    return ((bi & 1u) << 2) | ((bj & 1u) << 1) | (bk & 1u);
+}
+
+// The walk items grouped by brick colour, for the deterministic p2g: one
+// workgroup counts the colours, scans the eight counts and scatters the
+// items, renumbering each item's walk index within its colour.
+// colour[0..7] receives the counts, colour[8..15] the first item of each
+// colour in `out`.  (The order inside a colour follows the LDS atomics and
+// does not matter: one colour's bricks touch disjoint nodes.)
+template <typename G>
+__global__ __launch_bounds__(1024) void k_colour_items(G g, BrickItem* out, uint32_t* colour)
+{
+    __shared__ uint32_t cnt[8], cur[8];
+    if (threadIdx.x < 8) cnt[threadIdx.x] = cur[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t nw = walk_count(g);
+    for (uint32_t t = threadIdx.x; t < nw; t += blockDim.x)
+        atomicAdd(&cnt[brick_colour(g, g.items[t].b)], 1u);
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        uint32_t off = 0;
+        for (uint32_t c = 0; c < threadIdx.x; ++c) off += cnt[c];
+        colour[threadIdx.x] = cnt[threadIdx.x];
+        colour[8 + threadIdx.x] = off;
+    }
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < nw; t += blockDim.x) {
+        BrickItem it = g.items[t];
+        const uint32_t c = brick_colour(g, it.b);
+        const uint32_t slot = atomicAdd(&cur[c], 1u);
+        it.t = slot;
+        out[colour[8 + c] + slot] = it;
     }
 }
 
@@ -373,7 +422,7 @@ inline int brick_edge(const pic_binning& geom)
 // while a step graph of the context is alive).
 template <typename G>
 inline int setup_walk(pic_context* ctx, int order, int entity, int64_t np, const pic_bins& bins,
-                      G& g)
+                      G& g, size_t extra_bytes = 0, void** extra = nullptr)
 {
     const int bb = brick_edge(bins.geom);
     if (!bb || !bins.keys || !bins.bin_start || !bins.perm) return PIC_ERR_UNSUPPORTED;
@@ -391,10 +440,13 @@ inline int setup_walk(pic_context* ctx, int order, int entity, int64_t np, const
         g.nb[d] = (bins.geom.ncell[d] + bb - 1) / bb;
     }
     // scratch: items[nbricks] | walk counters | stale count | stale slots[np]
+    // | the caller's extra bytes (16-byte aligned)
     const size_t ibytes = size_t(nb) * sizeof(BrickItem);
     const size_t words = size_t(WALK_LANES) * WALK_PAD + 1 + size_t(np);
-    char* base = static_cast<char*>(scratch(ctx, ibytes + words * sizeof(uint32_t)));
+    const size_t wbytes = (words * sizeof(uint32_t) + 15) & ~size_t(15);
+    char* base = static_cast<char*>(scratch(ctx, ibytes + wbytes + extra_bytes));
     if (!base) return PIC_ERR_ALLOC;
+    if (extra) *extra = base + ibytes + wbytes;
     BrickItem* items = reinterpret_cast<BrickItem*>(base);
     uint32_t* w = reinterpret_cast<uint32_t*>(base + ibytes);
     g.items = items;

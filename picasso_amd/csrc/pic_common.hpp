@@ -13,7 +13,9 @@
 namespace pic {
 
 // Bits of the device-side error word latched in the context.
-enum : int { ERRBIT_BOUNDS = 1, ERRBIT_CELL = 2 };
+// ERRBIT_CAPACITY: a deterministic p2g had more stale particles than it can
+// order (its result is complete but not reproducible).
+enum : int { ERRBIT_BOUNDS = 1, ERRBIT_CELL = 2, ERRBIT_CAPACITY = 4 };
 
 // ---------------------------------------------------------------------------
 // Particle field: element (p, c) at data[(p >> L) * tile + (p & mask) + c * cs].

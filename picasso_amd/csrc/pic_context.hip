@@ -213,7 +213,7 @@ int pic_synchronize(pic_context* ctx)
     if (err) {
         PIC_HIP_CHECK(hipMemset(ctx->d_err, 0, sizeof(int)));
         PIC_HIP_CHECK(hipDeviceSynchronize());
-        return PIC_ERR_BOUNDS;
+        return (err & ~ERRBIT_CAPACITY) ? PIC_ERR_BOUNDS : PIC_ERR_CAPACITY;
     }
     return PIC_OK;
 }

@@ -769,6 +769,7 @@ extern "C" int pic_polypic_p2g(pic_context* ctx, int order, int entity, int dtyp
     if (bins && !bins->perm) return PIC_ERR_ARG;
     for (int d = 0; d < 3; ++d)
         if (mu_i->lo[d] != m_i->lo[d] || mu_i->n[d] != m_i->n[d]) return PIC_ERR_ARG;
+    if (flags & PIC_P2G_DETERMINISTIC) return PIC_ERR_UNSUPPORTED;  // APIC only
     DeviceGuard g(ctx->device);
     if (!g.ok) return PIC_ERR_HIP;
     const bool stag = stag_dim(entity) >= 0;
