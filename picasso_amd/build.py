@@ -31,6 +31,8 @@ COMPILE_FLAGS = [
     "-I" + os.path.join(HERE, "csrc"),
 ]
 LINK_FLAGS = ["--offload-arch=gfx950", "-shared", "-fPIC"]
+# roctx ranges around the entry points (rocprofv3 --marker-trace)
+LINK_LIBS = ["-L/opt/rocm/lib", "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"]
 
 
 def _jobs():
@@ -53,7 +55,7 @@ def build(verbose=False):
         print(" ".join([HIPCC] + COMPILE_FLAGS + ["-c", "<src>"]))
     with ThreadPoolExecutor(_jobs()) as ex:
         objs = list(ex.map(_compile, SOURCES))
-    subprocess.run([HIPCC] + LINK_FLAGS + objs + ["-o", LIB + ".tmp"], check=True)
+    subprocess.run([HIPCC] + LINK_FLAGS + objs + LINK_LIBS + ["-o", LIB + ".tmp"], check=True)
     os.replace(LIB + ".tmp", LIB)
     return LIB
 

@@ -195,6 +195,7 @@ extern "C" int pic_apic_p2g(pic_context* ctx, int order, int entity, int dtype,
                             const pic_grid* m_i, const pic_grid* mu_i,
                             const pic_bins* bins, uint32_t flags)
 {
+    Region region("picasso_amd::APIC::p2g");
     if (!ctx || !mesh || !(mesh->h > 0.0) || order < 1 || order > 3 ||
         !valid_entity(entity) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
         np > int64_t(UINT32_MAX) || !valid_grid(m_i) || !valid_grid(mu_i) ||
@@ -288,6 +289,7 @@ extern "C" int pic_apic_g2p(pic_context* ctx, int order, int entity, int dtype,
                             const pic_grid* u_i, const pic_pfield* c_p,
                             const pic_bins* bins)
 {
+    Region region("picasso_amd::APIC::g2p");
     return apic_g2p_impl(ctx, order, entity, dtype, mesh, np, x, u_i, nullptr, c_p, bins);
 }
 
@@ -296,6 +298,7 @@ extern "C" int pic_apic_g2p_normalized(pic_context* ctx, int order, int entity, 
                                        const pic_grid* m_i, const pic_grid* mu_i,
                                        const pic_pfield* c_p, const pic_bins* bins)
 {
+    Region region("picasso_amd::APIC::g2p_normalized");
     if (!m_i) return PIC_ERR_ARG;
     return apic_g2p_impl(ctx, order, entity, dtype, mesh, np, x, mu_i, m_i, c_p, bins);
 }
@@ -307,6 +310,7 @@ extern "C" int pic_apic_g2p_update_F(pic_context* ctx, int order, int entity, in
                                      const pic_pfield* det, const pic_pfield* Finv,
                                      const pic_bins* bins)
 {
+    Region region("picasso_amd::APIC::g2p_update_F");
     const FUpdate fu{F, det, Finv, dt};
     return apic_g2p_impl(ctx, order, entity, dtype, mesh, np, x, u_i, m_i, c_p, bins, &fu);
 }

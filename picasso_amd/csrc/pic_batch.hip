@@ -537,6 +537,7 @@ extern "C" int pic_batch_destroy(pic_batch* b)
 
 extern "C" int pic_batch_bin_sort(pic_context* ctx, pic_batch* b, const pic_pfield* x)
 {
+    Region region("picasso_amd::batch::bin_sort");
     if (!ctx || !b || b->device != ctx->device) return PIC_ERR_ARG;
     if (b->np > 0 && !valid_pfield(x)) return PIC_ERR_ARG;
     DeviceGuard dg(ctx->device);
@@ -563,6 +564,7 @@ extern "C" int pic_batch_bin_sort(pic_context* ctx, pic_batch* b, const pic_pfie
 extern "C" int pic_batch_apic_p2g(pic_context* ctx, pic_batch* b, const pic_pfield* x,
                                   const pic_pfield* m_p, const pic_pfield* c_p, uint32_t flags)
 {
+    Region region("picasso_amd::batch::APIC::p2g");
     if (!ctx || !b || b->device != ctx->device || !valid_step_fields(b, x, m_p, c_p, true))
         return PIC_ERR_ARG;
     if (flags & PIC_P2G_DETERMINISTIC) return PIC_ERR_UNSUPPORTED;  // single instances only
@@ -581,6 +583,7 @@ extern "C" int pic_batch_apic_p2g(pic_context* ctx, pic_batch* b, const pic_pfie
 extern "C" int pic_batch_apic_g2p_normalized(pic_context* ctx, pic_batch* b, const pic_pfield* x,
                                              const pic_pfield* c_p)
 {
+    Region region("picasso_amd::batch::APIC::g2p_normalized");
     if (!ctx || !b || b->device != ctx->device || !valid_step_fields(b, x, nullptr, c_p, false))
         return PIC_ERR_ARG;
     if (b->np == 0) return PIC_OK;
@@ -592,6 +595,7 @@ extern "C" int pic_batch_apic_g2p_normalized(pic_context* ctx, pic_batch* b, con
 extern "C" int pic_batch_apic_step(pic_context* ctx, pic_batch* b, const pic_pfield* x,
                                    const pic_pfield* m_p, const pic_pfield* c_p, uint32_t flags)
 {
+    Region region("picasso_amd::batch::APIC::step");
     if (const int rc = pic_batch_bin_sort(ctx, b, x)) return rc;
     if (const int rc = pic_batch_apic_p2g(ctx, b, x, m_p, c_p, flags)) return rc;
     return pic_batch_apic_g2p_normalized(ctx, b, x, c_p);

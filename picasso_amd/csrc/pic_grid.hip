@@ -143,6 +143,7 @@ using namespace pic;
 extern "C" int pic_grid_normalize(pic_context* ctx, int dtype, int ncomp, const pic_grid* m_i,
                                   const pic_grid* mu_i, const pic_grid* u_i, double rel_mass_eps)
 {
+    Region region("picasso_amd::grid_normalize");
     if (!ctx || (dtype != PIC_F64 && dtype != PIC_F32) || ncomp < 1 || ncomp > 3 ||
         !valid_grid(m_i) || !valid_grid(mu_i) || !valid_grid(u_i) || rel_mass_eps < 0.0)
         return PIC_ERR_ARG;
@@ -183,6 +184,7 @@ extern "C" int pic_g2p_gradu(pic_context* ctx, int order, int entity, int dtype,
                              const pic_mesh* mesh, int64_t np, const pic_pfield* x,
                              const pic_grid* u_i, const pic_pfield* gradu, const pic_bins* bins)
 {
+    Region region("picasso_amd::G2P::gradu");
     if (!ctx || !mesh || !(mesh->h > 0.0) || order < 1 || order > 3 ||
         (entity != PIC_NODE && entity != PIC_CELL) || (dtype != PIC_F64 && dtype != PIC_F32) ||
         np < 0 || np > int64_t(UINT32_MAX) || !valid_grid(u_i))
@@ -212,6 +214,7 @@ extern "C" int pic_update_F(pic_context* ctx, int64_t np, int gradu_dtype, const
                             double dt, const pic_pfield* F, const pic_pfield* det,
                             const pic_pfield* Finv)
 {
+    Region region("picasso_amd::update_F");
     if (!ctx || np < 0 || (gradu_dtype != PIC_F64 && gradu_dtype != PIC_F32)) return PIC_ERR_ARG;
     if (np == 0) return PIC_OK;
     if (!valid_pfield(gradu) || !valid_pfield(F) || !valid_pfield(det) || !valid_pfield(Finv))

@@ -79,6 +79,7 @@ extern "C" int pic_grid_halo(pic_context* ctx, int op, int dtype, const pic_grid
                              const int32_t own_lo[3], const int32_t own_n[3],
                              const int32_t periodic[3])
 {
+    Region region("picasso_amd::grid_halo");
     if (!ctx || !valid_grid(g) || !own_lo || !own_n || !periodic || ncomp < 1 ||
         (op != PIC_HALO_GATHER && op != PIC_HALO_SCATTER_SUM) ||
         (dtype != PIC_F64 && dtype != PIC_F32))
@@ -130,6 +131,7 @@ extern "C" int pic_grid_operator_apply(pic_context* ctx, int dtype, int ngather,
                                        const pic_field_dep* scatter, pic_apply_fn body,
                                        void* user)
 {
+    Region region("Picasso::GridOperator::apply");
     if (!ctx || !body || ngather < 0 || nscatter < 0 || (ngather > 0 && !gather) ||
         (nscatter > 0 && !scatter) || (dtype != PIC_F64 && dtype != PIC_F32))
         return PIC_ERR_ARG;

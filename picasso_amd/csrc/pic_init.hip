@@ -78,6 +78,7 @@ extern "C" int64_t pic_init_particles(pic_context* ctx, int mode, int dtype,
                                       int64_t capacity, const pic_pfield* x,
                                       const pic_pfield* volume)
 {
+    Region region("Picasso::initializeParticles");
     if (!ctx || !mesh || !(mesh->h > 0.0) || !cell_lo || !ncell ||
         (mode != PIC_INIT_UNIFORM && mode != PIC_INIT_RANDOM) ||
         (dtype != PIC_F64 && dtype != PIC_F32) || ppc < 1 || capacity < 0)

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <rocprofiler-sdk-roctx/roctx.h>
 
 #include <stdint.h>
 
@@ -52,6 +53,16 @@ namespace pic {
         hipError_t e_ = (expr);                            \
         if (e_ != hipSuccess) return PIC_ERR_HIP;          \
     } while (0)
+
+// A roctx range around one library call (rocprofv3 --marker-trace shows
+// it): named after the function it replaces, the GridOperator apply with the
+// reference's own region name (Picasso_GridOperator.hpp:464).
+struct Region {
+    explicit Region(const char* name) { roctxRangePush(name); }
+    ~Region() { roctxRangePop(); }
+    Region(const Region&) = delete;
+    Region& operator=(const Region&) = delete;
+};
 
 // Make the context's device current for the duration of a call.
 struct DeviceGuard {

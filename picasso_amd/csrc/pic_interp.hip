@@ -522,6 +522,7 @@ extern "C" int pic_g2p_interp(pic_context* ctx, int op, int order, int entity, i
                               const pic_grid* field, int ncomp, const pic_pfield* out,
                               const pic_bins* bins)
 {
+    Region region("picasso_amd::G2P");
     return interp_run(ctx, true, op, order, entity, dtype, mesh, np, x, out, ncomp, field, 0u,
                       bins);
 }
@@ -531,6 +532,7 @@ extern "C" int pic_p2g_interp(pic_context* ctx, int op, int order, int entity, i
                               const pic_pfield* value, int ncomp, const pic_grid* field,
                               uint32_t flags, const pic_bins* bins)
 {
+    Region region("picasso_amd::P2G");
     return interp_run(ctx, false, op, order, entity, dtype, mesh, np, x, value, ncomp, field,
                       flags, bins);
 }

@@ -758,6 +758,7 @@ extern "C" int pic_polypic_p2g(pic_context* ctx, int order, int entity, int dtyp
                                const pic_grid* mu_i, const pic_grid* m_i, double dt,
                                const pic_bins* bins, uint32_t flags)
 {
+    Region region("picasso_amd::PolyPIC::p2g");
     if (!ctx || !mesh || !(mesh->h > 0.0) || order < 1 || order > 3 ||
         !valid_entity(entity) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
         np > int64_t(UINT32_MAX) || ncomp < 1 || ncomp > 3 || !valid_grid(mu_i) ||
@@ -809,6 +810,7 @@ extern "C" int pic_polypic_g2p(pic_context* ctx, int order, int entity, int dtyp
                                const pic_pfield* x, const pic_grid* u_i,
                                const pic_pfield* c_p, const pic_bins* bins)
 {
+    Region region("picasso_amd::PolyPIC::g2p");
     if (!ctx || !mesh || !(mesh->h > 0.0) || order < 1 || order > 3 ||
         !valid_entity(entity) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
         np > int64_t(UINT32_MAX) || ncomp < 1 || ncomp > 3 || !valid_grid(u_i))

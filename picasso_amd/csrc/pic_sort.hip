@@ -545,6 +545,7 @@ extern "C" int pic_bin_sort(pic_context* ctx, int dtype, const pic_mesh* mesh, i
                             const pic_pfield* x, const pic_binning* geom, uint32_t* keys,
                             uint32_t* perm, uint32_t* bin_start, uint32_t* active)
 {
+    Region region("picasso_amd::bin_sort");
     int64_t nb = 0;
     if (const int rc = check_sort_args(ctx, dtype, mesh, np, x, geom, keys, perm, bin_start, &nb))
         return rc;
@@ -578,6 +579,7 @@ extern "C" int pic_permute(pic_context* ctx, int dtype, int64_t np, const uint32
                            int nfields, const pic_pfield* src, const pic_pfield* dst,
                            const int* ncomp)
 {
+    Region region("picasso_amd::permute");
     if (!ctx || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 || nfields < 0 ||
         (nfields > 0 && (!src || !dst || !ncomp)))
         return PIC_ERR_ARG;
@@ -616,6 +618,7 @@ __global__ void k_keep_counts(int64_t np, const uint8_t* keep, uint32_t* count)
 
 extern "C" int pic_select(pic_context* ctx, int64_t np, const uint8_t* keep, uint32_t* kept)
 {
+    Region region("picasso_amd::select");
     if (!ctx || np < 0 || np >= int64_t(UINT32_MAX) || !kept || (np > 0 && !keep))
         return PIC_ERR_ARG;
     DeviceGuard dg(ctx->device);
@@ -675,6 +678,7 @@ extern "C" int pic_bin_sort_noalloc(pic_context* ctx, int dtype, const pic_mesh*
                                     const pic_pfield* x, const pic_binning* geom, uint32_t* keys,
                                     uint32_t* perm, uint32_t* bin_start, uint32_t* active)
 {
+    Region region("picasso_amd::bin_sort_noalloc");
     int64_t nb = 0;
     if (const int rc = check_sort_args(ctx, dtype, mesh, np, x, geom, keys, perm, bin_start, &nb))
         return rc;
