@@ -101,6 +101,11 @@ struct P2GShape {
     // group 0 holds (g0, g1, g2, m / 8), group 1 + d holds m U_d / 8 and
     // m h D^-1 B_da / 8.
     static constexpr bool FOLD = PRE && !DMA;
+    // FP32 quadratic (staged by direct copies): the position rows hold
+    // g = f + 1/2 and the mass row m / 8, for quad_weights2; a particle the
+    // brick skips is staged with zero mass, U_d and B_d (and g = 1/2), so the
+    // accumulation loop needs no per-particle test.
+    static constexpr bool G2 = PRE && DMA;
     static constexpr int ROW_M = 3, ROW_C = 4;
     static constexpr int STAGE = CAP * 16;                   // scalars
     static constexpr int SUMS = CPR * SLOTS;
@@ -237,6 +242,9 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
     const T hh = (T)a.lat.h;
     const T Dinv = T(ORDER == 3 ? 3.0 : 4.0) / (hh * hh);  // APIC::inertialScaling (:33-47)
     const T hDinv = hh * Dinv;
+    // Lane constants of the G2 loop: the momentum work-items use U and
+    // h D^-1 B; the mass work-item U * 0 + 1 and 0 * B.
+    const T us = comp > 0 ? T(1) : T(0), ut = T(1) - us, hDl = us * hDinv;
     T acc[NN];
 #pragma unroll
     for (int n = 0; n < NN; ++n) acc[n] = T(0);
@@ -271,8 +279,10 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
                 for (int e = 0; e < 12; ++e) load_to_lds(&a.c.at(cb, e), row + (S::ROW_C + e) * CAP);
             }
             T frk[K][3];
+            bool skip[K];
 #pragma unroll
             for (int k = 0; k < K; ++k) {
+                skip[k] = false;
                 const uint32_t s = c0 + threadIdx.x + uint32_t(k * NT);
                 if (s >= c1) continue;
                 const int q = int(s - c0);
@@ -280,6 +290,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
                 const int eb[3] = {f0[0] + lb / (PB * PB), f0[1] + (lb / PB) % PB, f0[2] + lb % PB};
                 T* fr = frk[k];
                 bool ok = true;
+                T gr[3];
 #pragma unroll
                 for (int dim = 0; dim < 3; ++dim) {
                     double fd;
@@ -287,23 +298,41 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
                                                          a.lat.rdx, fd);
                     ok = ok && int(kb0) + Knots<ORDER>::FIRST == eb[dim];
                     fr[dim] = T(fd);
+                    gr[dim] = S::G2 ? T(fd + 0.5) : T(fd);
                 }
                 if (!ok) {
                     push_stale(g, s);
-                    fr[0] = T(STALE_F);
+                    skip[k] = true;
+                    if constexpr (S::G2) {
+#pragma unroll
+                        for (int dim = 0; dim < 3; ++dim) {
+                            fr[dim] = T(0);
+                            gr[dim] = T(0.5);
+                        }
+                    } else {
+                        gr[0] = T(STALE_F);
+                    }
                 }
 #pragma unroll
-                for (int dim = 0; dim < 3; ++dim) lds[dim * CAP + q] = fr[dim];
+                for (int dim = 0; dim < 3; ++dim) lds[dim * CAP + q] = gr[dim];
                 st.cs.template note<BINS>(k, q, kb, rng);
             }
             __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);
             if constexpr (S::PRE) {
-                // U_d over v_d; this work-item's own copies have landed.
+                // U_d over v_d and m / 8 over m; this work-item's own copies
+                // have landed.
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const uint32_t s = c0 + threadIdx.x + uint32_t(k * NT);
                     if (s >= c1) continue;
                     T* cr = lds + S::ROW_C * CAP + int(s - c0);
+                    T* mr = lds + S::ROW_M * CAP + int(s - c0);
+                    if (skip[k]) {
+                        *mr = T(0);
+#pragma unroll
+                        for (int e = 0; e < 12; ++e) cr[e * CAP] = T(0);
+                        continue;
+                    }
                     const T* fr = frk[k];
 #pragma unroll
                     for (int d = 0; d < 3; ++d) {
@@ -311,6 +340,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
                                      cr[(9 + d) * CAP] * fr[2];
                         cr[d * CAP] = cr[d * CAP] - hDinv * sB;
                     }
+                    *mr = T(0.125) * *mr;
                 }
             }
         } else {
@@ -369,6 +399,24 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
         if (c0 == beg) w.fetch();
         const uint32_t q0 = rng[l], q1 = rng[BINS + l];
         for (uint32_t q = q0; q < q1; ++q) {
+            if constexpr (S::G2) {
+                // Rows: g0 g1 g2, m / 8, then U_d = row d and
+                // B(d, a) = c(1 + a, d) = row 3 + 3a + d of the c block.
+                // The mass work-item takes U = 1, B = 0 through its lane
+                // constants (us, ut, hDl).
+                const T g0 = lds[q], g1 = lds[CAP + q], g2 = lds[2 * CAP + q];
+                const T m8 = lds[S::ROW_M * CAP + q];
+                const T* cr = lds + (S::ROW_C + grp - 1) * CAP + q;
+                const T U = cr[0], B0 = cr[3 * CAP], B1 = cr[6 * CAP], B2 = cr[9 * CAP];
+                T wm[3], w1[3], w2[3];
+                quad_weights2(g0, wm);
+                quad_weights2(g1, w1);
+                quad_weights2(g2, w2);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) wm[i] *= m8;
+                p2g_component_pre(wm, w1, w2, U * us + ut, B0, B1, B2, hDl, acc);
+                continue;
+            }
             if constexpr (S::PRE) {
                 T4 fm, ub;
                 if constexpr (S::DMA) {
