@@ -8,7 +8,12 @@ properties that do not depend on size:
   returns the same c_p, and the grid velocity at every massive node is
   u(x_n) (Picasso_APIC.hpp:55-140, 407-472; linear precision of the
   splines);
-- p2g conserves mass and linear momentum.
+- p2g conserves mass and linear momentum;
+- the workload's own random state (v = rotation plus noise, random B_p)
+  against the oracle on a box of nodes across the sphere's surface: the
+  oracle is given every particle whose stencil reaches the box (its sums at
+  the box nodes are then complete) and reconstructs the particles whose
+  stencils lie inside it from the GPU's grid.
 FP64 within 1e-11, FP32 within 1e-5 (relative)."""
 import numpy as np
 import pytest
@@ -103,3 +108,52 @@ def test_full_size_affine_step_and_conservation(pa, wl, dtype):
     cg = gu.host(cd).astype(np.float64).reshape(P, 4, 3)
     assert gu.rel_inf(cg[:, 0], c[:, 0]) < 10 * tol
     assert gu.rel_inf(cg[:, 1:], c[:, 1:]) < 100 * tol
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_full_size_random_state_box_vs_oracle(pa, wl, dtype):
+    import gpu_util as gu
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    f32 = dtype == "f32"
+    tol = 1e-5 if f32 else 1e-11
+    P = wl.np
+    cast = (lambda a: a.astype(np.float32).astype(np.float64)) if f32 else (lambda a: a)
+    x, m, c = cast(wl.x), cast(wl.m), cast(wl.c.reshape(P, 12))
+    mesh = pa.Mesh(wl.low, wl.h)
+    xd, md, cd = gu.soa(x, tdt), gu.soa(m[:, None], tdt), gu.soa(c, tdt)
+    lo, n = wl.node_window
+    shp = tuple(int(k) for k in n)
+    gm = pa.GridField(torch.zeros(shp, dtype=tdt, device=gu.DEV))
+    gmu = pa.GridField(torch.zeros(shp + (3,), dtype=tdt, device=gu.DEV))
+    bins = pa.bin_sort(gu.ctx(), mesh, xd, _geom(pa, wl))
+    sd = pa.create_spline(pa.FieldLocation.Node, 2, mesh, xd, bins)
+    pa.APIC.p2g(gu.ctx(), md, cd, gm, gmu, sd, zero_first=True)
+    gu.ctx().synchronize()
+    # nodes [L, U): across the surface on the -x side of the sphere (the
+    # surface is at x = 0.2, node 2 + 51.2), centred in y and z
+    L = np.array([2 + 51 - 8, 2 + 128 - 8, 2 + 128 - 8])
+    U = L + 16
+    lo_w, n_w = L - 2, U - L + 4
+    win = tuple(slice(int(a), int(a + b)) for a, b in zip(lo_w, n_w))
+    gmh = gu.host(gm.tensor[win]).astype(np.float64)
+    gmuh = gu.host(gmu.tensor[win]).astype(np.float64)
+    t = (x - wl.low) / wl.h
+    first = np.floor(t + 0.5).astype(np.int64) - 1
+    touch = np.nonzero(np.all((first + 2 >= L) & (first < U), axis=1))[0]
+    inside = np.nonzero(np.all((first >= L) & (first + 2 < U), axis=1))[0]
+    assert len(inside) > 1000 and (gmh == 0).any() and (gmh > 0).any()
+    rm = np.zeros(tuple(n_w))
+    rmu = np.zeros(tuple(n_w) + (3,))
+    lo3, n3 = lo_w.astype(np.int32), n_w.astype(np.int32)
+    orpic.apic_p2g(2, orpic.NODE, wl.low, wl.h, lo3, n3, x[touch], m[touch],
+                   c[touch].reshape(-1, 4, 3), rm, rmu)
+    box = (slice(2, 18),) * 3
+    assert gu.rel_inf(gmh[box], rm[box]) < tol
+    assert gu.rel_inf(gmuh[box], rmu[box]) < tol
+    pa.APIC.g2p_normalized(gu.ctx(), gm, gmu, cd, sd)
+    gu.ctx().synchronize()
+    rc = orpic.apic_g2p(2, orpic.NODE, wl.low, wl.h, lo3, n3, x[inside],
+                        orpic.grid_normalize(gmh, gmuh))
+    cg = gu.host(cd[torch.as_tensor(inside, device=gu.DEV)]).astype(np.float64).reshape(-1, 4, 3)
+    assert gu.rel_inf(cg[:, 0], rc[:, 0]) < tol
+    assert gu.rel_inf(cg[:, 1:], rc[:, 1:]) < 10 * tol
