@@ -262,6 +262,8 @@ __device__ __forceinline__ void load_to_lds(const void* src, void* wave_dst)
 // s_waitcnt operand for vmcnt(0), other counters left free (gfx9 layout:
 // vmcnt bits 3:0 and 15:14, expcnt 6:4, lgkmcnt 11:8).
 constexpr int WAIT_VMCNT0 = 0x0F70;
+// s_waitcnt operand for lgkmcnt(0), vmcnt and expcnt left free.
+constexpr int WAIT_LGKMCNT0 = 0xC07F;
 
 template <typename G>
 __device__ __forceinline__ void push_stale(const G& g, uint32_t s)

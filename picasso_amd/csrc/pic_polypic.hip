@@ -349,7 +349,7 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
                                                       const BrickItem& it,
                                                       BrickWalk<true, PolyBrickArgs<T>>& w,
                                                       T* lds, uint32_t* pidx,
-                                                      PolyStage<ORDER, T>& st)
+                                                      PolyStage<ORDER, T>& st, float* mbuf)
 {
     using S = PolyBrickShape<ORDER, T>;
     using T4 = typename std::conditional<sizeof(T) == 4, float4, double4>::type;
@@ -387,8 +387,49 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
         const uint32_t q0 = rng[l], q1 = rng[BINS + l];
         if constexpr (SLAB) {
             const int il = comp;
+            // FP32 (MDMA): the 81 modes of a bin's next particle are copied
+            // from HBM into the wave's LDS buffer while the current one is
+            // summed, without registers: copy k takes mode 4 k + comp of the
+            // bin's particle (lane 4 l + comp of the wave), so the bin's
+            // work-items read modes 4 k .. 4 k + 3 back with one 16-byte
+            // load.  Otherwise each work-item loads the modes from HBM.
+            constexpr bool MDMA = sizeof(T) == 4;
+            constexpr int NCP = (3 * NM + 3) / 4;  // copies per particle
+            float* wbuf = mbuf + __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6) * (NCP * 64);
+            const int lw = (int(threadIdx.x) & 63) >> 2;  // the bin's position in the wave
+            auto copy_modes = [&](uint32_t qn) {
+                const T* src = &a.c.at(a.c.base(int64_t(pidx[qn])), comp);
+                const int64_t step = 4 * a.c.cs;
+#pragma unroll
+                for (int k = 0; k < NCP; ++k) {
+                    if (4 * k + comp < 3 * NM) load_to_lds(src, wbuf + 64 * k);
+                    src += step;
+                }
+            };
+            if constexpr (MDMA)
+                if (q0 < q1) copy_modes(q0);
             for (uint32_t qq = q0; qq < q1; ++qq) {
                 const int q = int(qq);
+                T C[3][E][E][E];
+                if constexpr (MDMA) {
+                    __builtin_amdgcn_s_waitcnt(WAIT_VMCNT0);  // this particle's modes
+#pragma unroll
+                    for (int k = 0; k < NCP; ++k) {
+                        const float4 v = *reinterpret_cast<const float4*>(wbuf + 64 * k + 4 * lw);
+                        const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                        for (int cc = 0; cc < 4; ++cc) {
+                            const int e = 4 * k + cc;
+                            if (e < 3 * NM)
+                                C[e % 3][MODE_EXP[e / 3][0]][MODE_EXP[e / 3][1]][MODE_EXP[e / 3][2]] =
+                                    T(vv[cc]);
+                        }
+                    }
+                    // the buffer is read before the next particle's copies
+                    // land in it
+                    __builtin_amdgcn_s_waitcnt(WAIT_LGKMCNT0);
+                    if (qq + 1 < q1) copy_modes(qq + 1);
+                }
                 const T4 fm = *reinterpret_cast<const T4*>(lds + stage_at(q, 0));
                 if (fm.x == T(STALE_F)) continue;
                 T ai[3][3];
@@ -400,14 +441,15 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
                     ai[r][2] = row.z;
                 }
                 // the modes of all three columns of c_p
-                T C[3][E][E][E];
-                const int64_t cb = a.c.base(int64_t(pidx[q]));
+                if constexpr (!MDMA) {
+                    const int64_t cb = a.c.base(int64_t(pidx[q]));
 #pragma unroll
-                for (int r = 0; r < NM; ++r)
+                    for (int r = 0; r < NM; ++r)
 #pragma unroll
-                    for (int dd = 0; dd < 3; ++dd)
-                        C[dd][MODE_EXP[r][0]][MODE_EXP[r][1]][MODE_EXP[r][2]] =
-                            a.c.at(cb, r * 3 + dd);
+                        for (int dd = 0; dd < 3; ++dd)
+                            C[dd][MODE_EXP[r][0]][MODE_EXP[r][1]][MODE_EXP[r][2]] =
+                                a.c.at(cb, r * 3 + dd);
+                }
                 const T fr[3] = {fm.x, fm.y, fm.z};
                 Spline<ORDER, T> sd;
                 spline_from_frac<ORDER, T, false, true>(a.lat, fr, sd);
@@ -683,10 +725,14 @@ __global__ __launch_bounds__((PolyBrickShape<ORDER, T>::NT), (PolyBrickShape<ORD
     __shared__ uint32_t pidx[S::CAP];
     __shared__ uint32_t rng[2 * 2 * S::BINS];
     __shared__ BrickItem q[2];
+    // FP32 cubic three-column p2g: per wave, the LDS copies of one
+    // particle's 81 modes for each of its 16 bins
+    constexpr int MB = ORDER == 3 && !STAG && sizeof(T) == 4 ? (S::NT / 64) * 21 * 64 : 4;
+    __shared__ __align__(16) float mbuf[MB];
     for (int i = threadIdx.x; i < 4 * S::BINS; i += S::NT) rng[i] = 0u;
     PolyStage<ORDER, T> st{rng, 0, false, {}};
     walk_bricks<true>(g, q, [&](const BrickItem& it, BrickWalk<true, PolyBrickArgs<T>>& w) {
-        polypic_p2g_one_brick<ORDER, T, STAG>(g, it, w, lds, pidx, st);
+        polypic_p2g_one_brick<ORDER, T, STAG>(g, it, w, lds, pidx, st, mbuf);
     });
 }
 
