@@ -639,67 +639,55 @@ __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
                 if constexpr (E > 2) F[q][2][n] = T(0.5) * sd.g2[q][n];
             }
         const int lbase = (li * FN + lj) * FN + lk;
-        T cm[E][E][E][NC];
-#pragma unroll
-        for (int e0 = 0; e0 < E; ++e0)
-#pragma unroll
-            for (int e1 = 0; e1 < E; ++e1)
-#pragma unroll
-                for (int e2 = 0; e2 < E; ++e2)
-#pragma unroll
-                    for (int d = 0; d < NC; ++d) cm[e0][e1][e2][d] = T(0);
-#pragma unroll
-        for (int i = 0; i < NK; ++i) {
-            T t[E][E][NC];
-#pragma unroll
-            for (int e1 = 0; e1 < E; ++e1)
-#pragma unroll
-                for (int e2 = 0; e2 < E; ++e2)
-#pragma unroll
-                    for (int d = 0; d < NC; ++d) t[e1][e2][d] = T(0);
-#pragma unroll
-            for (int j = 0; j < NK; ++j) {
-                T sk[E][NC];
-#pragma unroll
-                for (int e2 = 0; e2 < E; ++e2)
-#pragma unroll
-                    for (int d = 0; d < NC; ++d) sk[e2][d] = T(0);
-#pragma unroll
-                for (int k = 0; k < NK; ++k) {
-                    const int q = lbase + (i * FN + j) * FN + k;
-                    T ue[NC];
-#pragma unroll
-                    for (int d = 0; d < NC; ++d) ue[d] = us[d * FV + q];
-#pragma unroll
-                    for (int e2 = 0; e2 < E; ++e2)
-#pragma unroll
-                        for (int d = 0; d < NC; ++d) sk[e2][d] += F[2][e2][k] * ue[d];
-                }
-#pragma unroll
-                for (int e1 = 0; e1 < E; ++e1)
-#pragma unroll
-                    for (int e2 = 0; e2 < E; ++e2)
-#pragma unroll
-                        for (int d = 0; d < NC; ++d) t[e1][e2][d] += F[1][e1][j] * sk[e2][d];
-            }
+        const int64_t cb = a.c.base(p);
+        // One grid component at a time: its 27 sums in registers (the three
+        // at once held 190 VGPRs, two waves per SIMD; the loads and the
+        // arithmetic are the same either way).
+#pragma unroll 1
+        for (int d = 0; d < NC; ++d) {
+            const T* ud = us + d * FV + lbase;
+            T cm[E][E][E];
 #pragma unroll
             for (int e0 = 0; e0 < E; ++e0)
 #pragma unroll
                 for (int e1 = 0; e1 < E; ++e1)
 #pragma unroll
-                    for (int e2 = 0; e2 < E; ++e2)
+                    for (int e2 = 0; e2 < E; ++e2) cm[e0][e1][e2] = T(0);
 #pragma unroll
-                        for (int d = 0; d < NC; ++d) cm[e0][e1][e2][d] += F[0][e0][i] * t[e1][e2][d];
-        }
-        const int64_t cb = a.c.base(p);
+            for (int i = 0; i < NK; ++i) {
+                T t[E][E];
 #pragma unroll
-        for (int r = 0; r < NM; ++r) {
-            const auto& ce = cm[MODE_EXP[r][0]][MODE_EXP[r][1]][MODE_EXP[r][2]];
-            if constexpr (STAG) {
-                a.c.at(cb, r * a.pcols + a.col) = ce[0];
-            } else {
+                for (int e1 = 0; e1 < E; ++e1)
 #pragma unroll
-                for (int d = 0; d < 3; ++d) a.c.at(cb, r * 3 + d) = ce[d];
+                    for (int e2 = 0; e2 < E; ++e2) t[e1][e2] = T(0);
+#pragma unroll
+                for (int j = 0; j < NK; ++j) {
+                    T sk[E];
+#pragma unroll
+                    for (int e2 = 0; e2 < E; ++e2) sk[e2] = T(0);
+#pragma unroll
+                    for (int k = 0; k < NK; ++k) {
+                        const T ue = ud[(i * FN + j) * FN + k];
+#pragma unroll
+                        for (int e2 = 0; e2 < E; ++e2) sk[e2] += F[2][e2][k] * ue;
+                    }
+#pragma unroll
+                    for (int e1 = 0; e1 < E; ++e1)
+#pragma unroll
+                        for (int e2 = 0; e2 < E; ++e2) t[e1][e2] += F[1][e1][j] * sk[e2];
+                }
+#pragma unroll
+                for (int e0 = 0; e0 < E; ++e0)
+#pragma unroll
+                    for (int e1 = 0; e1 < E; ++e1)
+#pragma unroll
+                        for (int e2 = 0; e2 < E; ++e2) cm[e0][e1][e2] += F[0][e0][i] * t[e1][e2];
+            }
+#pragma unroll
+            for (int r = 0; r < NM; ++r) {
+                const T ce = cm[MODE_EXP[r][0]][MODE_EXP[r][1]][MODE_EXP[r][2]];
+                if constexpr (STAG) a.c.at(cb, r * a.pcols + a.col) = ce;
+                else a.c.at(cb, r * 3 + d) = ce;
             }
         }
     }
