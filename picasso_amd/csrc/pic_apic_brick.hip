@@ -77,13 +77,14 @@ __global__ __launch_bounds__(1024) void k_sort_stale(uint32_t* list, int* err)
 }
 
 template <int BB, int ORDER, typename T, bool GF, bool STAG>
-__global__ __launch_bounds__(BT) void k_apic_g2p_brick(BrickArgs<T> g)
+__global__ __launch_bounds__((G2PShape<T, GF, STAG>::NT)) void k_apic_g2p_brick(BrickArgs<T> g)
 {
+    using S = G2PShape<T, GF, STAG>;
     constexpr int FN = BB + ORDER;
     __shared__ T us[(STAG ? 1 : 3) * FN * FN * FN];
     __shared__ BrickItem q[2];
     walk_bricks<false>(g, q, [&](const BrickItem& it, BrickWalk<false, BrickArgs<T>>& w) {
-        g2p_one_brick<BB, ORDER, T, GF, STAG>(g, it, w, us);
+        g2p_one_brick<BB, ORDER, T, GF, STAG, S::NT, S::R>(g, it, w, us);
     });
 }
 
@@ -146,9 +147,10 @@ struct G2PBrickLaunch {
     template <int BB, int O, bool GF, bool STAG>
     void launch()
     {
+        constexpr int NT = G2PShape<T, GF, STAG>::NT;
         const unsigned nbk =
-            persistent_blocks(ctx, k_apic_g2p_brick<BB, O, T, GF, STAG>, BT, 0, g.nbricks);
-        k_apic_g2p_brick<BB, O, T, GF, STAG><<<nbk, BT, 0, ctx->stream>>>(g);
+            persistent_blocks(ctx, k_apic_g2p_brick<BB, O, T, GF, STAG>, NT, 0, g.nbricks);
+        k_apic_g2p_brick<BB, O, T, GF, STAG><<<nbk, NT, 0, ctx->stream>>>(g);
     }
     // The F update is collocated only (the caller rejects it otherwise).
     template <int BB, int O>

@@ -34,7 +34,21 @@
 
 namespace pic {
 
-constexpr int BT = 256;  // work-items per g2p brick workgroup
+constexpr int BT = 256;  // work-items per g2p brick workgroup (batches, FP64, F update)
+
+// Work-items per workgroup and particles per work-item at a time of a
+// single-instance g2p brick kernel.  FP32 collocated g2p without the F
+// update runs 128 work-items with four particles each: on the 256^3 sphere
+// g2p 0.287 -> 0.275 ms, the step 0.879 -> 0.871 ms.  The others are faster
+// at 256 x 3: FP64 (1.753 against 1.769 ms per step), the staggered
+// faces (MAC step 4.26 against 4.46 ms) and the batched FP32 g2p over 512
+// instances (6.39 against 6.57 ms).
+template <typename T, bool GF, bool STAG>
+struct G2PShape {
+    static constexpr bool SMALL = sizeof(T) == 4 && !GF && !STAG;
+    static constexpr int NT = SMALL ? 128 : BT;
+    static constexpr int R = SMALL ? 4 : 3;
+};
 constexpr int PB = 4;    // p2g brick edge in bins
 
 template <typename T>
@@ -510,7 +524,7 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
     }
 }
 
-template <int BB, int ORDER, typename T, bool GF, bool STAG>
+template <int BB, int ORDER, typename T, bool GF, bool STAG, int NT = BT, int R = 3>
 __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const BrickItem& it,
                                               BrickWalk<false, BrickArgs<T>>& w, T* us)
 {
@@ -522,7 +536,7 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const Brick
     const ApicArgs<T>& a = g.a;
     const GView<T>& gu = a.gmu;
     w.fetch();
-    for (int q = threadIdx.x; q < FV; q += BT) {
+    for (int q = threadIdx.x; q < FV; q += NT) {
         const int ni = f0[0] + q / (FN * FN), nj = f0[1] + (q / FN) % FN, nk = f0[2] + q % FN;
         const bool in = gu.covers(0, ni, ni) && gu.covers(1, nj, nj) && gu.covers(2, nk, nk);
         const int64_t off = in ? gu.off(ni, nj, nk) : 0;
@@ -549,25 +563,24 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const Brick
 
     // Up to R particles per work-item at a time, with all their position
     // loads issued before any result is stored.
-    constexpr int R = 3;
-    for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += R * BT) {
+    for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += R * NT) {
         int64_t pp[R];
         T xp[R][3];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const uint32_t s = s0 + uint32_t(r * BT);
+            const uint32_t s = s0 + uint32_t(r * NT);
             pp[r] = s < end ? int64_t(a.perm[s]) : 0;
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            if (s0 + uint32_t(r * BT) >= end) continue;
+            if (s0 + uint32_t(r * NT) >= end) continue;
             const int64_t xb = a.x.base(pp[r]);
 #pragma unroll
             for (int dim = 0; dim < 3; ++dim) xp[r][dim] = a.x.at(xb, dim);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const uint32_t s = s0 + uint32_t(r * BT);
+            const uint32_t s = s0 + uint32_t(r * NT);
             if (s >= end) continue;
             Spline<ORDER, T> sd;
             eval_spline<ORDER, T, GF, true>(a.lat, xp[r], sd);
