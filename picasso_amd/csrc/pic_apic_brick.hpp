@@ -562,18 +562,30 @@ __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const Brick
     __syncthreads();
 
     // Up to R particles per work-item at a time, with all their position
-    // loads issued before any result is stored.
+    // loads issued before any result is stored.  The positions are first
+    // loaded from the slot itself, beside the permutation entry, as if the
+    // particles were stored in bin order (the usual case, where perm[s] = s):
+    // then no position load waits for the permutation.  Slots whose entry
+    // differs load the position again from the particle it names.
     for (uint32_t s0 = beg + threadIdx.x; s0 < end; s0 += R * NT) {
         int64_t pp[R];
         T xp[R][3];
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t s = s0 + uint32_t(r * NT);
-            pp[r] = s < end ? int64_t(a.perm[s]) : 0;
+            if (s >= end) {
+                pp[r] = 0;
+                continue;
+            }
+            pp[r] = int64_t(a.perm[s]);
+            const int64_t xb = a.x.base(int64_t(s));
+#pragma unroll
+            for (int dim = 0; dim < 3; ++dim) xp[r][dim] = a.x.at(xb, dim);
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            if (s0 + uint32_t(r * NT) >= end) continue;
+            const uint32_t s = s0 + uint32_t(r * NT);
+            if (s >= end || pp[r] == int64_t(s)) continue;
             const int64_t xb = a.x.base(pp[r]);
 #pragma unroll
             for (int dim = 0; dim < 3; ++dim) xp[r][dim] = a.x.at(xb, dim);
