@@ -757,6 +757,16 @@ def main():
     calib_dst = torch.empty_like(calib)
     calib_dst.copy_(calib)
     torch.cuda.synchronize()
+    # The same copy timed: the HBM bandwidth a plain device copy reaches on
+    # this box (bytes read + written), quoted beside the 8 TB/s peak.
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        calib_dst.copy_(calib)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * CALIB_BYTES * 20 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    log(f"device copy: {copy_gbs:.0f} GB/s (read + write)")
     del calib, calib_dst
     x32, m32, c32 = wl.x.astype(np.float32), wl.m.astype(np.float32), wl.c.astype(np.float32)
     families = ["bin_sort", "zero_grid", "apic_p2g", "max_mass", "normalize", "apic_g2p"]
@@ -853,7 +863,8 @@ def main():
         roof = {"bound": "hbm", "kernel": dom, "achieved": gbs, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
                 "traffic": traffic["bytes_per_launch"] if traffic else None,
-                "alg_bytes_per_launch": per_kernel_bytes[dom]}
+                "alg_bytes_per_launch": per_kernel_bytes[dom],
+                "measured_copy_GBs": copy_gbs, "frac_of_measured_copy": gbs / copy_gbs}
         if traffic:
             roof["traffic_source"] = traffic["source"]
     else:
@@ -918,6 +929,7 @@ def main():
         "step_alg_bytes": B_alg32,
         "step_alg_GBs": step_gbs,
         "step_hbm_frac": step_gbs / HBM_PEAK_GBS,
+        "step_frac_of_measured_copy": step_gbs / copy_gbs,
         "kernels": kt32,
         "other_configs": others,
     }
