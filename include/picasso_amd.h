@@ -223,6 +223,26 @@ int pic_apic_g2p_update_F(pic_context* ctx, int order, int entity, int dtype,
                           const pic_pfield* det, const pic_pfield* Finv,
                           const pic_bins* bins);
 
+/* One APIC transfer step in one call (no reference function: a Picasso
+ * step is the application's own sequence of GridOperator apply calls whose
+ * particle loops call APIC::p2g and APIC::g2p, Picasso_GridOperator.hpp:
+ * 459-496, 520-544).  Same results as
+ *   pic_bin_sort(x by geom into keys, perm, bin_start, active);
+ *   pic_apic_p2g(..., m_i, mu_i, those bins, PIC_P2G_ZERO_FIRST);
+ *   pic_apic_g2p_normalized(..., m_i, mu_i, c_p, those bins);
+ * with fewer launches when geom has 4x4x4 bricks on the spline's knot
+ * lattice (the p2g brick kernels): the grid windows (when each is one dense
+ * range) are cleared by extra workgroups of the sort's key pass, beside the
+ * keys, and the sort's scan lists the brick walk once for both transfers.
+ * Other binnings run the three calls.  Arguments as in those calls; active
+ * may be NULL. */
+int pic_apic_step(pic_context* ctx, int order, int entity, int dtype,
+                  const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                  const pic_pfield* m_p, const pic_pfield* c_p,
+                  const pic_grid* m_i, const pic_grid* mu_i,
+                  const pic_binning* geom, uint32_t* keys, uint32_t* perm,
+                  uint32_t* bin_start, uint32_t* active);
+
 /* Number of PolyPIC modes per component for a spline order: 8 (linear, the
  * reference's), 27 (orders 2 and 3, the {1,y,y^2}^3 basis). */
 int pic_polypic_nmode(int order);

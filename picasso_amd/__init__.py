@@ -409,6 +409,35 @@ class APIC:
               "pic_apic_g2p_normalized")
 
     @staticmethod
+    def step(ctx, m_p, c_p, m_i, mu_i, sd, geom, out=None):
+        """One APIC step in one call (pic_apic_step): bin_sort of sd.x by
+        geom, p2g into m_i / mu_i with the grid zeroed first, and g2p of
+        mu_i / m_i into c_p -- bin_sort, APIC.p2g(zero_first=True) and
+        APIC.g2p_normalized with fewer launches.  Returns the bins (out, or
+        new arrays), which later calls may reuse as sd.bins."""
+        import torch
+        x = sd.x
+        np_, dt = x.shape[0], _dtype_code(x)
+        nb = bin_count(geom)
+        if out is None:
+            i32 = dict(dtype=torch.int32, device=x.device)
+            out = BinSort(torch.empty(np_, **i32), torch.empty(np_, **i32),
+                          torch.empty(nb + 1, **i32), geom, torch.empty(nb + 1, **i32))
+        _check_bins(out, np_, nb)
+        out.geom = geom
+        args = (ctypes.byref(sd.mesh.c()), np_, ctypes.byref(_pfield(x, 3, "x")),
+                ctypes.byref(_pfield(m_p, 1, "m_p", np_, dt)),
+                ctypes.byref(_pfield(c_p, 12, "c_p", np_, dt)),
+                ctypes.byref(_grid(m_i, "m_i", dt, 1)),
+                ctypes.byref(_grid(mu_i, "mu_i", dt, _ncomp_grid(sd.location, 3))),
+                ctypes.byref(geom.c()), out.keys.data_ptr(), out.perm.data_ptr(),
+                out.bin_start.data_ptr(),
+                out.active.data_ptr() if out.active is not None else None)
+        check(lib.pic_apic_step(ctx._bind_stream(), sd.order, sd.location, dt, *args),
+              "pic_apic_step")
+        return out
+
+    @staticmethod
     def g2p_update_F(ctx, u_i, c_p, dt, F, det, Finv, sd, m_i=None):
         """APIC::g2p fused with the velocity gradient and the FP64
         deformation-gradient update F <- (I + dt grad u) F, det(F), F^-1.

@@ -101,6 +101,9 @@ PROTOTYPES = {
                               _vp, _vp, _vp]),
     "pic_bin_sort_noalloc": (_c_int, [_vp, _c_int, _P(Mesh), _c_i64, _P(PField), _P(Binning),
                                       _vp, _vp, _vp, _vp]),
+    "pic_apic_step": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(Mesh), _c_i64, _P(PField),
+                               _P(PField), _P(PField), _P(Grid), _P(Grid), _P(Binning), _vp,
+                               _vp, _vp, _vp]),
     "pic_apic_g2p_normalized": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(Mesh), _c_i64,
                                          _P(PField), _P(Grid), _P(Grid), _P(PField), _P(Bins)]),
     "pic_apic_g2p_update_F": (_c_int, [_vp, _c_int, _c_int, _c_int, _P(Mesh), _c_i64,

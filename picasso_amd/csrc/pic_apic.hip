@@ -303,6 +303,42 @@ extern "C" int pic_apic_g2p_normalized(pic_context* ctx, int order, int entity, 
     return apic_g2p_impl(ctx, order, entity, dtype, mesh, np, x, mu_i, m_i, c_p, bins);
 }
 
+extern "C" int pic_apic_step(pic_context* ctx, int order, int entity, int dtype,
+                             const pic_mesh* mesh, int64_t np, const pic_pfield* x,
+                             const pic_pfield* m_p, const pic_pfield* c_p, const pic_grid* m_i,
+                             const pic_grid* mu_i, const pic_binning* geom, uint32_t* keys,
+                             uint32_t* perm, uint32_t* bin_start, uint32_t* active)
+{
+    Region region("picasso_amd::APIC::step");
+    if (!ctx || !mesh || !(mesh->h > 0.0) || order < 1 || order > 3 ||
+        !valid_entity(entity) || (dtype != PIC_F64 && dtype != PIC_F32) || np < 0 ||
+        np >= int64_t(UINT32_MAX) || !valid_grid(m_i) || !valid_grid(mu_i) ||
+        !same_window(*m_i, *mu_i) || !geom || !bin_start)
+        return PIC_ERR_ARG;
+    if (np > 0 && (!valid_pfield(x) || !valid_pfield(m_p) || !valid_pfield(c_p) || !keys ||
+                   !perm))
+        return PIC_ERR_ARG;
+    const int64_t nb = pic_bin_count(geom);
+    if (nb < 0) return int(nb);
+    {
+        DeviceGuard g(ctx->device);
+        if (!g.ok) return PIC_ERR_HIP;
+        const int rc = apic_step_bricks(ctx, order, entity, dtype, *mesh, np, *x, *m_p, *c_p,
+                                        *m_i, *mu_i, *geom, keys, perm, bin_start, active);
+        if (rc != PIC_ERR_UNSUPPORTED) return rc;
+    }
+    // Binnings the p2g bricks do not take, and empty particle sets: the
+    // three calls one after another.
+    if (const int rc = pic_bin_sort(ctx, dtype, mesh, np, x, geom, keys, perm, bin_start, active))
+        return rc;
+    const pic_bins bins{perm, keys, bin_start, active, *geom};
+    const pic_bins* bp = np > 0 ? &bins : nullptr;  // no particles: no perm either
+    if (const int rc = pic_apic_p2g(ctx, order, entity, dtype, mesh, np, x, m_p, c_p, m_i, mu_i,
+                                    bp, PIC_P2G_ZERO_FIRST))
+        return rc;
+    return pic_apic_g2p_normalized(ctx, order, entity, dtype, mesh, np, x, m_i, mu_i, c_p, bp);
+}
+
 extern "C" int pic_apic_g2p_update_F(pic_context* ctx, int order, int entity, int dtype,
                                      const pic_mesh* mesh, int64_t np, const pic_pfield* x,
                                      const pic_grid* m_i, const pic_grid* u_i,

@@ -328,6 +328,13 @@ bool apic_brick_supported(int order, int entity, const pic_bins& bins);
 int apic_g2p_brick(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
                    int64_t np, const pic_pfield& x, const pic_grid& gu, const pic_grid* gmass,
                    const pic_pfield& c, const pic_bins& bins, const FUpdate* fu);
+// Sort, zero-first p2g and normalised g2p through the bricks in one pass
+// of launches (pic_apic_step); PIC_ERR_UNSUPPORTED when the binning does not
+// suit the p2g brick kernels.
+int apic_step_bricks(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
+                     int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
+                     const pic_grid& gm, const pic_grid& gmu, const pic_binning& geom,
+                     uint32_t* keys, uint32_t* perm, uint32_t* bin_start, uint32_t* active);
 
 // Attach the F-update outputs to the arguments of a g2p.
 template <typename T>

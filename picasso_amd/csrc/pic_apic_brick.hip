@@ -229,6 +229,105 @@ static int g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
                          fu);
 }
 
+// One APIC step through the bricks (pic_apic_step): the sort with the grid
+// windows, the walk counters and both stale counts cleared by its key pass
+// and the walk items listed by its scan, then the p2g and normalised g2p
+// brick kernels and their stale lists.  Scratch: the sort's layout, then
+// walk counters | p2g stale list (1 + np words) | g2p stale list | items.
+template <typename T>
+static int step_transfers(pic_context* ctx, int order, int entity, int dtype,
+                          const pic_mesh& mesh, int64_t np, const pic_pfield& x,
+                          const pic_pfield& m, const pic_pfield& c, const pic_grid& gm,
+                          const pic_grid& gmu, const pic_bins& bins, uint32_t* ctr,
+                          uint32_t* stale_p2g, uint32_t* stale_g2p, const BrickItem* items)
+{
+    const bool stag = stag_dim(entity) >= 0;
+    BrickArgs<T> g;
+    if (const int rc = walk_geometry(order, entity, bins, g)) return rc;
+    g.items = items;
+    g.walk = ctr;
+    g.stale = stale_p2g;
+    g.a = make_args<T>(ctx, entity, mesh, np, &x, &m, &c, &gm, gmu, bins.perm);
+    {
+        TimedLaunch tl(ctx, "apic_p2g");
+        P2GBrickLaunch<T> L{ctx, g, stag};
+        L.by_order(order);
+        if (launch_status()) return PIC_ERR_HIP;
+    }
+    if (const int rc = apic_p2g_list(ctx, order, entity, dtype, mesh, np, x, m, c, gm, gmu,
+                                     bins.perm, stale_p2g, false))
+        return rc;
+    BrickArgs<T> h = g;
+    h.stale = stale_g2p;
+    h.a = make_args<T>(ctx, entity, mesh, np, &x, nullptr, &c, nullptr, gmu, bins.perm);
+    h.a.gnm = make_gview<T>(gm);
+    {
+        TimedLaunch tl(ctx, "apic_g2p");
+        G2PBrickLaunch<T> L{ctx, h, stag};
+        L.template by_order<PB>(order);
+        if (launch_status()) return PIC_ERR_HIP;
+    }
+    return apic_g2p_list(ctx, order, entity, dtype, mesh, np, x, gmu, &gm, c, bins.perm,
+                         stale_g2p, nullptr);
+}
+
+int apic_step_bricks(pic_context* ctx, int order, int entity, int dtype, const pic_mesh& mesh,
+                     int64_t np, const pic_pfield& x, const pic_pfield& m, const pic_pfield& c,
+                     const pic_grid& gm, const pic_grid& gmu, const pic_binning& geom,
+                     uint32_t* keys, uint32_t* perm, uint32_t* bin_start, uint32_t* active)
+{
+    int32_t knot_off[3];
+    const int64_t nb = pic_bin_count(&geom);
+    if (np <= 0 || nb <= 0 || brick_edge(geom) != PB ||
+        !aligned_knot_offset(order, entity, geom.dual, knot_off))
+        return PIC_ERR_UNSUPPORTED;
+    const SortLayout L = sort_layout(nb, np, active != nullptr);
+    const size_t sort_bytes = (L.bytes + 255) & ~size_t(255);
+    const size_t ctr_words = size_t(WALK_LANES) * WALK_PAD;
+    const size_t stale_words = 1 + size_t(np);
+    const size_t items_at =
+        (sort_bytes + (ctr_words + 2 * stale_words) * sizeof(uint32_t) + 15) & ~size_t(15);
+    char* base = static_cast<char*>(scratch(ctx, items_at + size_t(nb) * sizeof(BrickItem)));
+    if (!base) return PIC_ERR_ALLOC;
+    uint32_t* sc = reinterpret_cast<uint32_t*>(base);
+    uint32_t* ctr = reinterpret_cast<uint32_t*>(base + sort_bytes);
+    uint32_t* stale_p2g = ctr + ctr_words;
+    uint32_t* stale_g2p = stale_p2g + stale_words;
+    BrickItem* items = reinterpret_cast<BrickItem*>(base + items_at);
+
+    // What the key pass clears: dense grid windows (others get a fill of
+    // their own first), the walk counters and the two stale counts.
+    const size_t esz = dtype == PIC_F64 ? 8 : 4;
+    const int ncomp = stag_dim(entity) >= 0 ? 1 : 3;
+    ZeroJob z;
+    const pic_grid* grids[2] = {&gm, &gmu};
+    const int gcomp[2] = {1, ncomp};
+    for (int k = 0; k < 2; ++k) {
+        int64_t count = 0;
+        if (!(dense_window(*grids[k], gcomp[k], &count) &&
+              z.add(grids[k]->data, count * int64_t(esz)))) {
+            if (const int rc = zero_grid(ctx, dtype, *grids[k], gcomp[k])) return rc;
+        }
+    }
+    z.add(ctr, int64_t(ctr_words * sizeof(uint32_t)));
+    z.add(stale_p2g, sizeof(uint32_t));
+    z.add(stale_g2p, sizeof(uint32_t));
+
+    uint32_t* act = active ? active : sort_active(L, sc);
+    {
+        TimedLaunch tl(ctx, "bin_sort");
+        if (const int rc = bin_sort_with(ctx, dtype, mesh, np, &x, geom, nb, L, sc, keys, perm,
+                                         bin_start, active, z, items))
+            return rc;
+    }
+    const pic_bins bins{perm, keys, bin_start, act, geom};
+    if (dtype == PIC_F32)
+        return step_transfers<float>(ctx, order, entity, dtype, mesh, np, x, m, c, gm, gmu, bins,
+                                     ctr, stale_p2g, stale_g2p, items);
+    return step_transfers<double>(ctx, order, entity, dtype, mesh, np, x, m, c, gm, gmu, bins,
+                                  ctr, stale_p2g, stale_g2p, items);
+}
+
 bool apic_brick_supported(int order, int entity, const pic_bins& bins)
 {
     int32_t knot_off[3];

@@ -8,21 +8,16 @@
 
 #include "pic_common.hpp"
 #include "pic_internal.hpp"
+#include "pic_sort.hpp"
 
 namespace pic {
-
-// One brick of the walk: walk index t, brick id b and its slot range
-// [beg, end) in the sorted order.
-struct alignas(16) BrickItem {
-    uint32_t t, b, beg, end;
-};
 
 // Binning fields every brick kernel argument struct carries.
 #define PIC_BRICK_WALK_FIELDS                                                        \
     const uint32_t* keys;                                                            \
     const uint32_t* bin_start;                                                       \
     const uint32_t* active; /* [count, ids...] of non-empty bricks, or null */       \
-    const BrickItem* items; /* one per walk index, written by k_walk_items */        \
+    const BrickItem* items; /* one per walk index (k_walk_items or the sort's scan) */\
     uint32_t* walk;         /* WALK_LANES counters, WALK_PAD words apart, zero */    \
     uint32_t* stale;        /* [count, slots...] of particles for the slow path */   \
     uint32_t nbricks;                                                                \
@@ -418,13 +413,12 @@ inline int brick_edge(const pic_binning& geom)
     return (e == 4 || e == 8) ? e : 0;
 }
 
-// Fill the walk fields of g for bins.  PIC_ERR_UNSUPPORTED when the brick
+// The walk fields of g that follow from the bins (all but the scratch
+// pointers items, walk and stale).  PIC_ERR_UNSUPPORTED when the brick
 // kernels do not apply (the caller then falls back to the per-particle
-// kernels), PIC_ERR_ALLOC when the scratch cannot grow (inside a capture, or
-// while a step graph of the context is alive).
+// kernels).
 template <typename G>
-inline int setup_walk(pic_context* ctx, int order, int entity, int64_t np, const pic_bins& bins,
-                      G& g, size_t extra_bytes = 0, void** extra = nullptr)
+inline int walk_geometry(int order, int entity, const pic_bins& bins, G& g)
 {
     const int bb = brick_edge(bins.geom);
     if (!bb || !bins.keys || !bins.bin_start || !bins.perm) return PIC_ERR_UNSUPPORTED;
@@ -441,6 +435,19 @@ inline int setup_walk(pic_context* ctx, int order, int entity, int64_t np, const
         g.cell_lo[d] = bins.geom.cell_lo[d];
         g.nb[d] = (bins.geom.ncell[d] + bb - 1) / bb;
     }
+    return PIC_OK;
+}
+
+// Fill the walk fields of g for bins and list the walk items in scratch.
+// PIC_ERR_UNSUPPORTED when the brick kernels do not apply, PIC_ERR_ALLOC
+// when the scratch cannot grow (inside a capture, or while a step graph of
+// the context is alive).
+template <typename G>
+inline int setup_walk(pic_context* ctx, int order, int entity, int64_t np, const pic_bins& bins,
+                      G& g, size_t extra_bytes = 0, void** extra = nullptr)
+{
+    if (const int rc = walk_geometry(order, entity, bins, g)) return rc;
+    const int64_t nb = int64_t(g.nbricks);
     // scratch: items[nbricks] | walk counters | stale count | stale slots[np]
     // | the caller's extra bytes (16-byte aligned)
     const size_t ibytes = size_t(nb) * sizeof(BrickItem);

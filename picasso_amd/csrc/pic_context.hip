@@ -82,8 +82,7 @@ __global__ __launch_bounds__(256) void k_zero_flat(uint4* p, int64_t n16, char* 
     if (blockIdx.x == 0 && threadIdx.x < ntail) tail[threadIdx.x] = 0;
 }
 
-// Do the window's elements (ncomp components) cover one dense range?
-static bool dense_window(const pic_grid& g, int ncomp, int64_t* count)
+bool dense_window(const pic_grid& g, int ncomp, int64_t* count)
 {
     int64_t ext[4] = {g.n[0], g.n[1], g.n[2], ncomp};
     int64_t str[4] = {g.stride[0], g.stride[1], g.stride[2], ncomp > 1 ? g.cstride : 0};

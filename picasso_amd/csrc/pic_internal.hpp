@@ -144,4 +144,8 @@ int noalloc_setup(int device);
 // Zero a grid window on the stream (used by PIC_P2G_ZERO_FIRST).
 int zero_grid(pic_context* ctx, int dtype, const pic_grid& g, int ncomp);
 
+// Do the window's elements (ncomp components) cover one dense range?  Its
+// element count goes to *count.
+bool dense_window(const pic_grid& g, int ncomp, int64_t* count);
+
 }  // namespace pic

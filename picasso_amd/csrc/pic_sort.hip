@@ -155,10 +155,12 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_sums(uint64_t* sums, int6
     }
 }
 
+// With items not null, non-empty brick t of the list also gets its walk
+// item {t, brick, start, end}.
 __global__ __launch_bounds__(SCAN_THREADS) void k_scan_compact(const uint32_t* count, int64_t nb,
                                                                const uint64_t* sums,
                                                                bool prefixed, uint32_t* bin_start,
-                                                               uint32_t* active)
+                                                               uint32_t* active, BrickItem* items)
 {
     // offset of this tile: the sum of the tiles before it
     uint64_t tile_off;
@@ -184,7 +186,13 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_compact(const uint32_t* c
         const int64_t i = base + q;
         if (i <= nb) {
             bin_start[i] = uint32_t(run);
-            if (v[q] >> 32) active[1 + uint32_t(run >> 32)] = uint32_t(i);
+            if (v[q] >> 32) {
+                const uint32_t t = uint32_t(run >> 32);
+                active[1 + t] = uint32_t(i);
+                if (items)
+                    items[t] = BrickItem{t, uint32_t(i), uint32_t(run),
+                                         uint32_t(run) + uint32_t(v[q])};
+            }
         }
         run += v[q];
     }
@@ -485,15 +493,20 @@ void sort_clear(pic_context* ctx, const SortLayout& L, uint32_t* sc)
     k_fill_u32<<<fb, 256, 0, ctx->stream>>>(sc, L.nzero, 0u);
 }
 
+uint32_t* sort_active(const SortLayout& L, uint32_t* sc)
+{
+    return reinterpret_cast<uint32_t*>(reinterpret_cast<uint64_t*>(sc + L.nzero) + L.ntile);
+}
+
 int sort_finish(pic_context* ctx, int64_t np, int64_t nb, uint32_t bvol, int32_t vshift,
                 const SortLayout& L, uint32_t* sc, uint32_t* keys, uint32_t* perm,
-                uint32_t* bin_start, uint32_t* active)
+                uint32_t* bin_start, uint32_t* active, BrickItem* items)
 {
     uint32_t* count = sc;
     uint32_t* cursor = count + nb + 1;
     uint32_t* unsorted = cursor + nb + 1;
     uint64_t* tile_sums = reinterpret_cast<uint64_t*>(sc + L.nzero);
-    uint32_t* act = active ? active : reinterpret_cast<uint32_t*>(tile_sums + L.ntile);
+    uint32_t* act = active ? active : sort_active(L, sc);
     uint32_t* tmp = sc + L.head;
     uint32_t* tmpk = tmp + np;
     uint32_t* big = tmpk + np;
@@ -502,7 +515,7 @@ int sort_finish(pic_context* ctx, int64_t np, int64_t nb, uint32_t bvol, int32_t
     const bool prefixed = L.ntile > SCAN_TILE;
     if (prefixed) k_scan_sums<<<1, SCAN_THREADS, 0, st>>>(tile_sums, L.ntile);
     k_scan_compact<<<(unsigned)L.ntile, SCAN_THREADS, 0, st>>>(count, nb, tile_sums, prefixed,
-                                                               bin_start, act);
+                                                               bin_start, act, items);
     if (np > 0) {
         k_bin_scatter<<<blocks_for(np, SORT_PT * SORT_ITEMS), SORT_PT, 0, st>>>(
             np, keys, vshift, bvol, uint32_t(nb), bin_start, unsorted, cursor, tmp, tmpk, perm);
@@ -515,28 +528,40 @@ int sort_finish(pic_context* ctx, int64_t np, int64_t nb, uint32_t bvol, int32_t
 }
 
 // The sort's passes on the context's stream with scratch `sc` laid out by
-// sort_layout (device memory of at least L.bytes).
-static int bin_sort_run(pic_context* ctx, int dtype, const pic_mesh& mesh, int64_t np,
-                        const pic_pfield* x, const pic_binning& geom, int64_t nb,
-                        const SortLayout& L, uint32_t* sc, uint32_t* keys, uint32_t* perm,
-                        uint32_t* bin_start, uint32_t* active)
+// sort_layout (device memory of at least L.bytes), the clearing job z run
+// beside the key pass and, when items is not null, the walk items listed
+// by the scan.
+int bin_sort_with(pic_context* ctx, int dtype, const pic_mesh& mesh, int64_t np,
+                  const pic_pfield* x, const pic_binning& geom, int64_t nb, const SortLayout& L,
+                  uint32_t* sc, uint32_t* keys, uint32_t* perm, uint32_t* bin_start,
+                  uint32_t* active, const ZeroJob& z, BrickItem* items)
 {
     const KeyGeom g = make_key_geom(mesh, geom);
     sort_clear(ctx, L, sc);
-    if (np > 0) {
-        const unsigned b = blocks_for(np, SORT_PT * SORT_ITEMS);
+    if (np > 0 || z.blocks > 0) {
+        const unsigned b = key_blocks(np) + z.blocks;
         uint32_t* count = sort_count(sc);
         uint32_t* unsorted = sort_unsorted(sc, nb);
         if (dtype == PIC_F64)
             k_cell_keys<<<b, SORT_PT, 0, ctx->stream>>>(
                 np, MeshKeys<double>{make_pfield<double>(*x), g}, uint32_t(nb), keys, count,
-                unsorted, ctx->d_err);
+                unsorted, ctx->d_err, z);
         else
             k_cell_keys<<<b, SORT_PT, 0, ctx->stream>>>(
                 np, MeshKeys<float>{make_pfield<float>(*x), g}, uint32_t(nb), keys, count,
-                unsorted, ctx->d_err);
+                unsorted, ctx->d_err, z);
     }
-    return sort_finish(ctx, np, nb, g.bvol, g.vshift, L, sc, keys, perm, bin_start, active);
+    return sort_finish(ctx, np, nb, g.bvol, g.vshift, L, sc, keys, perm, bin_start, active,
+                       items);
+}
+
+static int bin_sort_run(pic_context* ctx, int dtype, const pic_mesh& mesh, int64_t np,
+                        const pic_pfield* x, const pic_binning& geom, int64_t nb,
+                        const SortLayout& L, uint32_t* sc, uint32_t* keys, uint32_t* perm,
+                        uint32_t* bin_start, uint32_t* active)
+{
+    return bin_sort_with(ctx, dtype, mesh, np, x, geom, nb, L, sc, keys, perm, bin_start, active,
+                         ZeroJob{}, nullptr);
 }
 
 }  // namespace pic
@@ -639,7 +664,7 @@ extern "C" int pic_select(pic_context* ctx, int64_t np, const uint8_t* keep, uin
     const bool prefixed = ntile > SCAN_TILE;
     if (prefixed) k_scan_sums<<<1, SCAN_THREADS, 0, st>>>(tile_sums, ntile);
     k_scan_compact<<<(unsigned)ntile, SCAN_THREADS, 0, st>>>(count, np, tile_sums, prefixed, scan,
-                                                             kept);
+                                                             kept, nullptr);
     return launch_status();
 }
 

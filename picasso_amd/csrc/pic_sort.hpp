@@ -22,6 +22,63 @@ struct KeyGeom {
 
 KeyGeom make_key_geom(const pic_mesh& mesh, const pic_binning& b);
 
+// One brick of a walk over the non-empty bricks: walk index t, brick id b
+// and its slot range [beg, end) in the sorted order.
+struct alignas(16) BrickItem {
+    uint32_t t, b, beg, end;
+};
+
+// Byte ranges that a key pass clears beside its keys, for a caller that
+// would otherwise clear them with launches of its own (pic_apic_step clears
+// the grid windows and the brick walk's counters this way, so the clearing
+// overlaps the key pass instead of following it).  Range r is bytes
+// [0, bytes[r]) from base[r]; its 16-byte words (base 16-byte aligned) are
+// shared out over `blocks` extra workgroups interleaved with the key
+// workgroups, `per` words each, and the last bytes[r] % 16 bytes by the
+// first of those workgroups.
+struct ZeroJob {
+    static constexpr int MAXR = 6;
+    static constexpr int64_t PER = 1024;  // 16-byte words per workgroup (16 KB)
+    char* base[MAXR];
+    int64_t bytes[MAXR];
+    int nr = 0;
+    uint32_t blocks = 0;
+    int64_t words = 0;
+    int64_t per = PER;
+
+    // false when full or when base is not 16-byte aligned for its words
+    bool add(void* p, int64_t nbytes)
+    {
+        if (nr == MAXR || (nbytes >= 16 && (reinterpret_cast<uintptr_t>(p) & 15) != 0))
+            return false;
+        base[nr] = static_cast<char*>(p);
+        bytes[nr] = nbytes;
+        ++nr;
+        words += nbytes >> 4;
+        blocks = uint32_t((words + PER - 1) / PER);
+        if (blocks == 0) blocks = 1;  // the tails alone
+        return true;
+    }
+};
+
+// Clearing workgroup zi of a ZeroJob: words [zi per, (zi + 1) per) of the
+// ranges laid end to end, and the tails when zi is 0.
+__device__ __forceinline__ void zero_share(const ZeroJob& z, uint32_t zi)
+{
+    const int64_t w0 = int64_t(zi) * z.per, w1 = w0 + z.per;
+    const uint4 zero = make_uint4(0u, 0u, 0u, 0u);
+    int64_t r0 = 0;
+    for (int r = 0; r < z.nr; ++r) {
+        const int64_t n = z.bytes[r] >> 4;
+        const int64_t a = w0 > r0 ? w0 : r0, b = w1 < r0 + n ? w1 : r0 + n;
+        uint4* p = reinterpret_cast<uint4*>(z.base[r]);
+        for (int64_t w = a + threadIdx.x; w < b; w += blockDim.x) p[w - r0] = zero;
+        if (zi == 0 && int64_t(threadIdx.x) < (z.bytes[r] & 15))
+            z.base[r][(n << 4) + threadIdx.x] = 0;
+        r0 += n;
+    }
+}
+
 // Key of particle p (0xffffffff when outside the bins); *brick receives the
 // brick index.
 template <typename T>
@@ -83,9 +140,21 @@ SortLayout sort_layout(int64_t nb, int64_t np, bool caller_active);
 inline uint32_t* sort_count(uint32_t* sc) { return sc; }
 inline uint32_t* sort_unsorted(uint32_t* sc, int64_t nb) { return sc + 2 * (nb + 1); }
 void sort_clear(pic_context* ctx, const SortLayout& L, uint32_t* sc);
+// items (may be null): the walk items of the non-empty bricks, written by
+// the scan for brick kernels that would otherwise list them in a prologue.
 int sort_finish(pic_context* ctx, int64_t np, int64_t nb, uint32_t bvol, int32_t vshift,
                 const SortLayout& L, uint32_t* sc, uint32_t* keys, uint32_t* perm,
-                uint32_t* bin_start, uint32_t* active);
+                uint32_t* bin_start, uint32_t* active, BrickItem* items = nullptr);
+// The active-brick list the sort keeps in its scratch when the caller
+// passes none.
+uint32_t* sort_active(const SortLayout& L, uint32_t* sc);
+// The whole sort of a mesh's particles (pic_bin_sort) with a clearing job
+// run beside its key pass and, when items is not null, the walk items.
+int bin_sort_with(pic_context* ctx, int dtype, const pic_mesh& mesh, int64_t np,
+                  const pic_pfield* x, const pic_binning& geom, int64_t nb, const SortLayout& L,
+                  uint32_t* sc, uint32_t* keys, uint32_t* perm, uint32_t* bin_start,
+                  uint32_t* active, const ZeroJob& z, BrickItem* items);
+
 
 // Wave-aggregated count: atomicAdd of 1 per active lane on base[key], lanes
 // with equal keys sharing one atomic.  Nothing is returned, so the atomics
@@ -110,6 +179,12 @@ __device__ __forceinline__ void wave_agg_count(uint32_t* base, uint32_t key, boo
 constexpr int SORT_ITEMS = 4;
 constexpr int SORT_PT = 256;  // work-items per block of those passes
 
+// Key-pass workgroups for np particles (a clearing job adds its own).
+inline unsigned key_blocks(int64_t np)
+{
+    return unsigned((np + SORT_PT * SORT_ITEMS - 1) / (SORT_PT * SORT_ITEMS));
+}
+
 // Keys of one mesh's particles: kf(p, &brick) is cell_key.
 template <typename T>
 struct MeshKeys {
@@ -130,14 +205,28 @@ struct MeshKeys {
 // keys[] in input order.  *unsorted is raised (by one atomic per
 // workgroup) when some particle's key is smaller than its predecessor's;
 // the predecessor of a workgroup's first particle is recomputed from its
-// position.
+// position.  With a clearing job (z.blocks > 0) the grid holds z.blocks
+// more workgroups: workgroup w clears when floor((w + 1) Z / W) exceeds
+// floor(w Z / W) (W workgroups, Z of them clearing), which spreads the
+// clearing evenly over the key pass; the others take the key workgroups in
+// order.
 template <typename KeyFn>
 __global__ __launch_bounds__(SORT_PT) void k_cell_keys(int64_t np, KeyFn kf, uint32_t nbricks,
                                                        uint32_t* keys, uint32_t* count,
-                                                       uint32_t* unsorted, int* err)
+                                                       uint32_t* unsorted, int* err, ZeroJob z)
 {
     __shared__ uint32_t skey[SORT_PT * SORT_ITEMS];
-    const int64_t blk0 = int64_t(blockIdx.x) * (SORT_PT * SORT_ITEMS);
+    uint32_t kblk = blockIdx.x;
+    if (z.blocks) {
+        const uint64_t W = gridDim.x, Z = z.blocks;
+        const uint64_t before = uint64_t(blockIdx.x) * Z / W;
+        if ((uint64_t(blockIdx.x) + 1) * Z / W > before) {
+            zero_share(z, uint32_t(before));
+            return;
+        }
+        kblk = blockIdx.x - uint32_t(before);
+    }
+    const int64_t blk0 = int64_t(kblk) * (SORT_PT * SORT_ITEMS);
     const int64_t p0 = blk0 + threadIdx.x;
     kf.begin(blk0);
     uint32_t key[SORT_ITEMS], b[SORT_ITEMS];
