@@ -136,6 +136,21 @@ inline void g2p_normalized(pic_context* ctx, const pic_grid& m_i, const pic_grid
                                   &m_i, &mu_i, &c_p, sd.bins));
 }
 
+// One step in one call (pic_apic_step): the bin sort of sd's particles by
+// geom into keys / perm / bin_start / active, p2g into m_i / mu_i with the
+// grid zeroed first and g2p of mu_i / m_i into c_p, with fewer launches than
+// binSort + p2g + g2p_normalized.  sd.bins is not used; the bins the step
+// wrote are returned.
+inline pic_bins step(pic_context* ctx, const pic_pfield& m_p, const pic_pfield& c_p,
+                     const pic_grid& m_i, const pic_grid& mu_i, const SplineData& sd,
+                     const pic_binning& geom, uint32_t* keys, uint32_t* perm,
+                     uint32_t* bin_start, uint32_t* active)
+{
+    check(pic_apic_step(ctx, sd.order, sd.entity, sd.dtype, &sd.mesh, sd.np, &sd.x, &m_p, &c_p,
+                        &m_i, &mu_i, &geom, keys, perm, bin_start, active));
+    return pic_bins{perm, keys, bin_start, active, geom};
+}
+
 // g2p fused with the velocity gradient and the FP64 update of F (and its
 // determinant and inverse).  m_i non-null: u_i is the grid momentum.
 inline void g2p_update_F(pic_context* ctx, const pic_grid* m_i, const pic_grid& u_i,

@@ -39,8 +39,9 @@ class StepConfig:
 class MovingStep:
     """One APIC step of a moving particle set (FP32 or FP64, quadratic Node
     splines): sort (every ``resort`` steps), p2g with the grid zeroed first,
-    g2p of the normalised grid velocity, advection x += dt v, and the
-    particle arrays stored in bin order every ``permute`` steps.
+    g2p of the normalised grid velocity (the three in one pic_apic_step call
+    on the steps that sort with 4^3-bin g2p bricks), advection x += dt v,
+    and the particle arrays stored in bin order every ``permute`` steps.
 
     x, m, c are [P, q] tensors (any strides); ``geom`` bins the quadratic
     knot base with 4^3-bin bricks; ``geom8`` the same cells in 8^3 bricks
@@ -68,14 +69,19 @@ class MovingStep:
 
     def run(self):
         pa, ctx, cf = self.pa, self.ctx, self.config
-        if self.step_no % cf.resort == 0:
-            pa.bin_sort(ctx, self.mesh, self.x, self.geom, out=self.bins)
-        sd = pa.create_spline(pa.FieldLocation.Node, 2, self.mesh, self.x, self.bins)
-        pa.APIC.p2g(ctx, self.m, self.c, self.gm, self.gmu, sd, zero_first=True)
-        if cf.g2p_brick == 8:
-            pa.bin_sort(ctx, self.mesh, self.x, self.geom8, out=self.bins8)
-            sd = pa.create_spline(pa.FieldLocation.Node, 2, self.mesh, self.x, self.bins8)
-        pa.APIC.g2p_normalized(ctx, self.gm, self.gmu, self.c, sd)
+        if self.step_no % cf.resort == 0 and cf.g2p_brick == 4:
+            # sort, p2g and g2p in one call (pic_apic_step)
+            sd = pa.create_spline(pa.FieldLocation.Node, 2, self.mesh, self.x)
+            pa.APIC.step(ctx, self.m, self.c, self.gm, self.gmu, sd, self.geom, out=self.bins)
+        else:
+            if self.step_no % cf.resort == 0:
+                pa.bin_sort(ctx, self.mesh, self.x, self.geom, out=self.bins)
+            sd = pa.create_spline(pa.FieldLocation.Node, 2, self.mesh, self.x, self.bins)
+            pa.APIC.p2g(ctx, self.m, self.c, self.gm, self.gmu, sd, zero_first=True)
+            if cf.g2p_brick == 8:
+                pa.bin_sort(ctx, self.mesh, self.x, self.geom8, out=self.bins8)
+                sd = pa.create_spline(pa.FieldLocation.Node, 2, self.mesh, self.x, self.bins8)
+            pa.APIC.g2p_normalized(ctx, self.gm, self.gmu, self.c, sd)
         self.x.add_(self.c[:, :3], alpha=self.dt)
         self.step_no += 1
         if cf.permute is not None and self.step_no % cf.permute == 0:
