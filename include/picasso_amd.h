@@ -492,6 +492,14 @@ int pic_batch_apic_g2p_normalized(pic_context* ctx, pic_batch* batch, const pic_
 int pic_batch_apic_step(pic_context* ctx, pic_batch* batch, const pic_pfield* x,
                         const pic_pfield* m_p, const pic_pfield* c_p, uint32_t flags);
 
+/* Tuning options of a context (no reference counterpart; picasso_amd/tune.py
+ * picks them on a live workload).  PIC_OPT_G2P_GROUP: work-items per
+ * workgroup of the collocated APIC g2p brick kernels without the F update,
+ * 128 (four particles per work-item at a time) or 256 (three); 0 restores
+ * the default (128 for FP32, 256 for FP64).  Results do not depend on it. */
+enum { PIC_OPT_G2P_GROUP = 1 };
+int pic_set_option(pic_context* ctx, int option, int value);
+
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on
  * the context's stream and pic_kernel_time() returns the accumulated
  * milliseconds and launch count of the named kernel family

@@ -122,6 +122,13 @@ class Context:
     def set_timing(self, on):
         check(lib.pic_set_timing(self.handle, int(bool(on))), "pic_set_timing")
 
+    def set_g2p_group(self, work_items):
+        """Work-items per workgroup of the collocated APIC g2p brick kernels
+        (pic_set_option PIC_OPT_G2P_GROUP): 128, 256, or 0 for the default
+        of each precision.  Results do not depend on it."""
+        check(lib.pic_set_option(self.handle, _abi.OPT_G2P_GROUP, int(work_items)),
+              "pic_set_option")
+
     def reset_timing(self):
         check(lib.pic_reset_timing(self.handle), "pic_reset_timing")
 

@@ -23,6 +23,7 @@ NODE, CELL, FACE_I, FACE_J, FACE_K, EDGE_I, EDGE_J, EDGE_K = range(8)
 F64, F32 = 0, 1
 P2G_ZERO_FIRST = 1
 P2G_DETERMINISTIC = 2
+OPT_G2P_GROUP = 1  # pic_set_option: work-items per collocated g2p brick workgroup
 
 
 class Mesh(ctypes.Structure):
@@ -125,6 +126,7 @@ PROTOTYPES = {
     "pic_batch_apic_step": (_c_int, [_vp, _vp, _P(PField), _P(PField), _P(PField), _c_u32]),
     "pic_grid_operator_apply": (_c_int, [_vp, _c_int, _c_int, _P(FieldDep), _c_int,
                                          _P(FieldDep), APPLY_FN, _vp]),
+    "pic_set_option": (_c_int, [_vp, _c_int, _c_int]),
     "pic_graph_begin": (_c_int, [_vp]),
     "pic_graph_end": (_c_int, [_vp, _P(_vp)]),
     "pic_graph_launch": (_c_int, [_vp, _vp]),

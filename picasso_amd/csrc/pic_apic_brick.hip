@@ -76,15 +76,14 @@ __global__ __launch_bounds__(1024) void k_sort_stale(uint32_t* list, int* err)
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) list[1 + i] = s[i];
 }
 
-template <int BB, int ORDER, typename T, bool GF, bool STAG>
-__global__ __launch_bounds__((G2PShape<T, GF, STAG>::NT)) void k_apic_g2p_brick(BrickArgs<T> g)
+template <int BB, int ORDER, typename T, bool GF, bool STAG, int NT, int R>
+__global__ __launch_bounds__(NT) void k_apic_g2p_brick(BrickArgs<T> g)
 {
-    using S = G2PShape<T, GF, STAG>;
     constexpr int FN = BB + ORDER;
     __shared__ T us[(STAG ? 1 : 3) * FN * FN * FN];
     __shared__ BrickItem q[2];
     walk_bricks<false>(g, q, [&](const BrickItem& it, BrickWalk<false, BrickArgs<T>>& w) {
-        g2p_one_brick<BB, ORDER, T, GF, STAG, S::NT, S::R>(g, it, w, us);
+        g2p_one_brick<BB, ORDER, T, GF, STAG, NT, R>(g, it, w, us);
     });
 }
 
@@ -144,13 +143,24 @@ struct G2PBrickLaunch {
     pic_context* ctx;
     const BrickArgs<T>& g;
     bool stag;
+    template <int BB, int O, bool GF, bool STAG, int NT, int R>
+    void launch_shape()
+    {
+        const unsigned nbk =
+            persistent_blocks(ctx, k_apic_g2p_brick<BB, O, T, GF, STAG, NT, R>, NT, 0, g.nbricks);
+        k_apic_g2p_brick<BB, O, T, GF, STAG, NT, R><<<nbk, NT, 0, ctx->stream>>>(g);
+    }
+    // The collocated kernel without the F update runs either workgroup
+    // shape (pic_set_option); the others keep their measured one.
     template <int BB, int O, bool GF, bool STAG>
     void launch()
     {
-        constexpr int NT = G2PShape<T, GF, STAG>::NT;
-        const unsigned nbk =
-            persistent_blocks(ctx, k_apic_g2p_brick<BB, O, T, GF, STAG>, NT, 0, g.nbricks);
-        k_apic_g2p_brick<BB, O, T, GF, STAG><<<nbk, NT, 0, ctx->stream>>>(g);
+        using S = G2PShape<T, GF, STAG>;
+        if constexpr (!GF && !STAG) {
+            if (ctx->g2p_group == 128) return launch_shape<BB, O, GF, STAG, 128, 4>();
+            if (ctx->g2p_group == 256) return launch_shape<BB, O, GF, STAG, BT, 3>();
+        }
+        launch_shape<BB, O, GF, STAG, S::NT, S::R>();
     }
     // The F update is collocated only (the caller rejects it otherwise).
     template <int BB, int O>

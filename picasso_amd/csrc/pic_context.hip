@@ -217,6 +217,17 @@ int pic_synchronize(pic_context* ctx)
     return PIC_OK;
 }
 
+int pic_set_option(pic_context* ctx, int option, int value)
+{
+    if (!ctx) return PIC_ERR_ARG;
+    if (option == PIC_OPT_G2P_GROUP) {
+        if (value != 0 && value != 128 && value != 256) return PIC_ERR_ARG;
+        ctx->g2p_group = value;
+        return PIC_OK;
+    }
+    return PIC_ERR_ARG;
+}
+
 int pic_set_timing(pic_context* ctx, int enable)
 {
     if (!ctx) return PIC_ERR_ARG;

@@ -42,6 +42,10 @@ struct pic_context {
     // any is alive.
     int live_graphs = 0;
 
+    // Work-items per workgroup of the collocated g2p brick kernels (0: the
+    // measured default of each precision; pic_set_option).
+    int g2p_group = 0;
+
     // Resident workgroups per compute unit, per (kernel, dynamic LDS bytes).
     std::map<std::pair<const void*, size_t>, int> occupancy;
 };

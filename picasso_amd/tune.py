@@ -14,11 +14,14 @@ costs against each other:
 - ``g2p_brick``: the brick edge of the g2p binning: 4 reuses the p2g bins,
   8 sorts a second time into 8^3-bin bricks whose footprints share more
   nodes.
+- ``g2p_group``: particles per g2p workgroup pass, as the workgroup size
+  (128 work-items with four particles each, or 256 with three; the
+  context option PIC_OPT_G2P_GROUP).
 
 ``StepTuner.tune`` times every candidate over a short window of real steps
 on the live workload (HIP events around the window) and keeps the fastest.
-Particle-per-workgroup counts and the p2g brick shape are compile-time
-constants of the kernels (4^3-bin bricks, the staging capacity), not tuned
+The p2g brick shape and its staging capacity are compile-time constants of
+the p2g kernels (4^3-bin bricks, 616 FP32 particles per chunk), not tuned
 here.
 """
 import itertools
@@ -31,9 +34,11 @@ class StepConfig:
     resort: int = 1
     permute: Optional[int] = None
     g2p_brick: int = 4
+    g2p_group: int = 0  # 0: the library's default for the precision
 
     def as_dict(self):
-        return {"resort": self.resort, "permute": self.permute, "g2p_brick": self.g2p_brick}
+        return {"resort": self.resort, "permute": self.permute, "g2p_brick": self.g2p_brick,
+                "g2p_group": self.g2p_group}
 
 
 class MovingStep:
@@ -69,6 +74,7 @@ class MovingStep:
 
     def run(self):
         pa, ctx, cf = self.pa, self.ctx, self.config
+        ctx.set_g2p_group(cf.g2p_group)
         if self.step_no % cf.resort == 0 and cf.g2p_brick == 4:
             # sort, p2g and g2p in one call (pic_apic_step)
             sd = pa.create_spline(pa.FieldLocation.Node, 2, self.mesh, self.x)
@@ -94,9 +100,10 @@ class StepTuner:
     candidate over `window` steps (after `warmup`) on the live particles,
     which keep moving throughout."""
 
-    def __init__(self, resort=(1, 2), permute=(1, 4, 16, None), g2p_brick=(4, 8)):
-        self.candidates = [StepConfig(r, p, g)
-                           for r, p, g in itertools.product(resort, permute, g2p_brick)]
+    def __init__(self, resort=(1, 2), permute=(1, 4, 16, None), g2p_brick=(4, 8),
+                 g2p_group=(128, 256)):
+        self.candidates = [StepConfig(r, p, g, w) for r, p, g, w in
+                           itertools.product(resort, permute, g2p_brick, g2p_group)]
 
     def tune(self, step, window=8, warmup=2):
         import torch
