@@ -551,11 +551,11 @@ extern "C" int pic_batch_bin_sort(pic_context* ctx, pic_batch* b, const pic_pfie
         if (b->dtype == PIC_F64)
             k_cell_keys<<<blk, SORT_PT, 0, ctx->stream>>>(
                 b->np, BatchKeys<double>{make_pfield<double>(*x), b->inst, b->first, b->ninst, 64, 0},
-                uint32_t(b->nb), b->keys, count, unsorted, ctx->d_err, ZeroJob{});
+                uint32_t(b->nb), b->keys, count, unsorted, ctx->d_err);
         else
             k_cell_keys<<<blk, SORT_PT, 0, ctx->stream>>>(
                 b->np, BatchKeys<float>{make_pfield<float>(*x), b->inst, b->first, b->ninst, 64, 0},
-                uint32_t(b->nb), b->keys, count, unsorted, ctx->d_err, ZeroJob{});
+                uint32_t(b->nb), b->keys, count, unsorted, ctx->d_err);
     }
     return sort_finish(ctx, b->np, b->nb, 64, 6, b->L, b->sort_sc, b->keys, b->perm, b->bin_start,
                        b->active);

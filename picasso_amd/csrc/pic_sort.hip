@@ -538,18 +538,28 @@ int bin_sort_with(pic_context* ctx, int dtype, const pic_mesh& mesh, int64_t np,
 {
     const KeyGeom g = make_key_geom(mesh, geom);
     sort_clear(ctx, L, sc);
-    if (np > 0 || z.blocks > 0) {
+    uint32_t* count = sort_count(sc);
+    uint32_t* unsorted = sort_unsorted(sc, nb);
+    if (z.blocks > 0) {
         const unsigned b = key_blocks(np) + z.blocks;
-        uint32_t* count = sort_count(sc);
-        uint32_t* unsorted = sort_unsorted(sc, nb);
         if (dtype == PIC_F64)
-            k_cell_keys<<<b, SORT_PT, 0, ctx->stream>>>(
+            k_cell_keys_clear<<<b, SORT_PT, 0, ctx->stream>>>(
                 np, MeshKeys<double>{make_pfield<double>(*x), g}, uint32_t(nb), keys, count,
                 unsorted, ctx->d_err, z);
         else
-            k_cell_keys<<<b, SORT_PT, 0, ctx->stream>>>(
+            k_cell_keys_clear<<<b, SORT_PT, 0, ctx->stream>>>(
                 np, MeshKeys<float>{make_pfield<float>(*x), g}, uint32_t(nb), keys, count,
                 unsorted, ctx->d_err, z);
+    } else if (np > 0) {
+        const unsigned b = key_blocks(np);
+        if (dtype == PIC_F64)
+            k_cell_keys<<<b, SORT_PT, 0, ctx->stream>>>(
+                np, MeshKeys<double>{make_pfield<double>(*x), g}, uint32_t(nb), keys, count,
+                unsorted, ctx->d_err);
+        else
+            k_cell_keys<<<b, SORT_PT, 0, ctx->stream>>>(
+                np, MeshKeys<float>{make_pfield<float>(*x), g}, uint32_t(nb), keys, count,
+                unsorted, ctx->d_err);
     }
     return sort_finish(ctx, np, nb, g.bvol, g.vshift, L, sc, keys, perm, bin_start, active,
                        items);

@@ -205,28 +205,12 @@ struct MeshKeys {
 // keys[] in input order.  *unsorted is raised (by one atomic per
 // workgroup) when some particle's key is smaller than its predecessor's;
 // the predecessor of a workgroup's first particle is recomputed from its
-// position.  With a clearing job (z.blocks > 0) the grid holds z.blocks
-// more workgroups: workgroup w clears when floor((w + 1) Z / W) exceeds
-// floor(w Z / W) (W workgroups, Z of them clearing), which spreads the
-// clearing evenly over the key pass; the others take the key workgroups in
-// order.
+// position.  blk0: the workgroup's first particle.
 template <typename KeyFn>
-__global__ __launch_bounds__(SORT_PT) void k_cell_keys(int64_t np, KeyFn kf, uint32_t nbricks,
-                                                       uint32_t* keys, uint32_t* count,
-                                                       uint32_t* unsorted, int* err, ZeroJob z)
+__device__ __forceinline__ void cell_keys_block(int64_t blk0, int64_t np, KeyFn& kf,
+                                                uint32_t nbricks, uint32_t* keys, uint32_t* count,
+                                                uint32_t* unsorted, int* err, uint32_t* skey)
 {
-    __shared__ uint32_t skey[SORT_PT * SORT_ITEMS];
-    uint32_t kblk = blockIdx.x;
-    if (z.blocks) {
-        const uint64_t W = gridDim.x, Z = z.blocks;
-        const uint64_t before = uint64_t(blockIdx.x) * Z / W;
-        if ((uint64_t(blockIdx.x) + 1) * Z / W > before) {
-            zero_share(z, uint32_t(before));
-            return;
-        }
-        kblk = blockIdx.x - uint32_t(before);
-    }
-    const int64_t blk0 = int64_t(kblk) * (SORT_PT * SORT_ITEMS);
     const int64_t p0 = blk0 + threadIdx.x;
     kf.begin(blk0);
     uint32_t key[SORT_ITEMS], b[SORT_ITEMS];
@@ -266,6 +250,41 @@ __global__ __launch_bounds__(SORT_PT) void k_cell_keys(int64_t np, KeyFn kf, uin
         descent = descent || key[k] < prev;
     }
     if (__syncthreads_or(descent) && threadIdx.x == 0) atomicOr(unsorted, 1u);
+}
+
+template <typename KeyFn>
+__global__ __launch_bounds__(SORT_PT) void k_cell_keys(int64_t np, KeyFn kf, uint32_t nbricks,
+                                                       uint32_t* keys, uint32_t* count,
+                                                       uint32_t* unsorted, int* err)
+{
+    __shared__ uint32_t skey[SORT_PT * SORT_ITEMS];
+    cell_keys_block(int64_t(blockIdx.x) * (SORT_PT * SORT_ITEMS), np, kf, nbricks, keys, count,
+                    unsorted, err, skey);
+}
+
+// The key pass with a clearing job: the grid holds z.blocks more
+// workgroups, and workgroup w clears when floor((w + 1) Z / W) exceeds
+// floor(w Z / W) (W workgroups, Z of them clearing), which spreads the
+// clearing evenly over the key pass; the others take the key workgroups in
+// order.  A kernel of its own, so that the plain key pass keeps its
+// workgroup index on the scalar unit (a batch's per-workgroup instance
+// search runs on scalar loads there).
+template <typename KeyFn>
+__global__ __launch_bounds__(SORT_PT) void k_cell_keys_clear(int64_t np, KeyFn kf,
+                                                             uint32_t nbricks, uint32_t* keys,
+                                                             uint32_t* count, uint32_t* unsorted,
+                                                             int* err, ZeroJob z)
+{
+    __shared__ uint32_t skey[SORT_PT * SORT_ITEMS];
+    const uint64_t W = gridDim.x, Z = z.blocks;
+    const uint64_t before = uint64_t(blockIdx.x) * Z / W;
+    if ((uint64_t(blockIdx.x) + 1) * Z / W > before) {
+        zero_share(z, uint32_t(before));
+        return;
+    }
+    const uint32_t kblk = __builtin_amdgcn_readfirstlane(blockIdx.x - uint32_t(before));
+    cell_keys_block(int64_t(kblk) * (SORT_PT * SORT_ITEMS), np, kf, nbricks, keys, count,
+                    unsorted, err, skey);
 }
 
 }  // namespace pic
