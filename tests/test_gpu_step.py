@@ -233,3 +233,47 @@ def test_step_graph_replays(pa, wl, dtype):
             assert gu.rel_inf(gu.host(c), ref[2]) < tol * 10
         graph.close()
         ctx.close()
+
+
+def test_step_without_active_list(pa, wl):
+    """active may be NULL: the step keeps the list of non-empty bricks in its
+    scratch, as pic_bin_sort does."""
+    import gpu_util as gu
+    ctx = gu.ctx()
+    mesh = pa.Mesh(wl.low, wl.h)
+    x = gu.soa(wl.x)
+    m = gu.soa(wl.m[:, None])
+    c = gu.soa(wl.c.reshape(-1, 12))
+    geom = _geom(pa, wl, 2, pa.NODE)
+    nb = pa.bin_count(geom)
+    i32 = dict(dtype=torch.int32, device=gu.DEV)
+    out = pa.BinSort(torch.empty(wl.np, **i32), torch.empty(wl.np, **i32),
+                     torch.empty(nb + 1, **i32), geom, None)
+    gm, gmu = _grids(pa, wl, torch.float64, 3)
+    pa.APIC.step(ctx, m, c, gm, gmu, pa.create_spline(pa.NODE, 2, mesh, x), geom, out=out)
+    ctx.synchronize()
+    rm, rmu, rc = _oracle(wl, 2, pa.NODE, wl.x, wl.m, wl.c)
+    assert gu.rel_inf(gu.host(gm.tensor), rm) < F64_TOL
+    assert gu.rel_inf(gu.host(gmu.tensor), rmu) < F64_TOL
+    assert gu.rel_inf(gu.host(c).reshape(-1, 4, 3), rc) < F64_TOL
+
+
+def test_step_reports_particle_outside_bins(pa, wl):
+    """A particle outside the binned cells is reported by synchronize, as
+    after pic_bin_sort (the outputs are then undefined)."""
+    import gpu_util as gu
+    ctx = gu.ctx()
+    mesh = pa.Mesh(wl.low, wl.h)
+    geom = _geom(pa, wl, 2, pa.NODE)
+    cell_lo, _ = wl.cell_region
+    xh = wl.x.copy()
+    # node coordinate 1.0: node-centred cell 1, one below the first binned cell (2)
+    xh[5, 0] = wl.low[0] + (int(cell_lo[0]) - 1.0) * wl.h
+    x = gu.soa(xh)
+    m = gu.soa(wl.m[:, None])
+    c = gu.soa(wl.c.reshape(-1, 12))
+    gm, gmu = _grids(pa, wl, torch.float64, 3)
+    pa.APIC.step(ctx, m, c, gm, gmu, pa.create_spline(pa.NODE, 2, mesh, x), geom)
+    with pytest.raises(pa.PicError):
+        ctx.synchronize()
+    ctx.synchronize()  # the error was cleared
