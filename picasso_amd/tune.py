@@ -69,8 +69,9 @@ class MovingStep:
         pa.permute(self.ctx, self.bins.perm, [(getattr(self, k), new[k]) for k in new])
         for k, v in new.items():
             setattr(self, k, v)
-        # the bins now describe the new storage order
-        self.bins = self.pa.bin_sort(self.ctx, self.mesh, self.x, self.geom, out=self.bins)
+        # Bins for the new storage order, unless the next step sorts anyway.
+        if self.step_no % self.config.resort != 0:
+            self.bins = pa.bin_sort(self.ctx, self.mesh, self.x, self.geom, out=self.bins)
 
     def run(self):
         pa, ctx, cf = self.pa, self.ctx, self.config
