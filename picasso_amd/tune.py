@@ -50,7 +50,9 @@ class MovingStep:
 
     x, m, c are [P, q] tensors (any strides); ``geom`` bins the quadratic
     knot base with 4^3-bin bricks; ``geom8`` the same cells in 8^3 bricks
-    (for ``g2p_brick = 8``)."""
+    (for ``g2p_brick = 8``).  Every step sets the context's g2p workgroup
+    option (``Context.set_g2p_group``) to the configuration's, and leaves it
+    so."""
 
     def __init__(self, pa, ctx, mesh, geom, geom8, x, m, c, gm, gmu, dt, config=StepConfig()):
         import torch
