@@ -86,6 +86,9 @@ __device__ __forceinline__ int instance_of(const int64_t* first, int ninst, int6
 // Key functor of the batch: instance k's local key and brick, moved to its
 // global bricks.  begin() finds the instance of the workgroup's first
 // particle; each particle then steps from there (instances are contiguous).
+// A workgroup whose particles all lie in that instance (all but the few
+// that straddle an instance boundary) skips the per-particle steps and
+// reads the instance's record with scalar loads.
 template <typename T>
 struct BatchKeys {
     PField<T> x;
@@ -94,8 +97,15 @@ struct BatchKeys {
     int ninst;
     uint32_t bvol;
     int k0;
+    bool one;      // particles [p_lo, p_lo + SORT_PT * SORT_ITEMS) all in instance k0
+    int64_t p_lo;
 
-    __device__ __forceinline__ void begin(int64_t p0) { k0 = instance_of(first, ninst, p0); }
+    __device__ __forceinline__ void begin(int64_t p0)
+    {
+        k0 = instance_of(first, ninst, p0);
+        p_lo = p0;
+        one = k0 + 1 >= ninst || first[k0 + 1] >= p0 + SORT_PT * SORT_ITEMS;
+    }
 
     __device__ __forceinline__ uint32_t key_in(const BatchInst& I, int64_t p, uint32_t* b) const
     {
@@ -109,6 +119,7 @@ struct BatchKeys {
 
     __device__ __forceinline__ uint32_t operator()(int64_t p, uint32_t* b) const
     {
+        if (one && p >= p_lo) return key_in(inst[k0], p, b);
         int k = k0;
         while (k > 0 && first[k] > p) --k;
         while (k + 1 < ninst && first[k + 1] <= p) ++k;
