@@ -170,8 +170,8 @@ k_batch_p2g_brick(BrickArgs<T> g, const BatchInst* __restrict__ inst, uint32_t s
     });
 }
 
-template <int ORDER, typename T>
-__global__ __launch_bounds__(BT) void k_batch_g2p_brick(BrickArgs<T> g,
+template <int ORDER, typename T, int NT, int R>
+__global__ __launch_bounds__(NT) void k_batch_g2p_brick(BrickArgs<T> g,
                                                         const BatchInst* __restrict__ inst,
                                                         uint32_t shift)
 {
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(BT) void k_batch_g2p_brick(BrickArgs<T> g,
         const uint32_t k = __builtin_amdgcn_readfirstlane(it.b) >> shift;
         BrickArgs<T> gi = g;
         instance_args(inst[k], gi, true);
-        g2p_one_brick<PB, ORDER, T, false, false>(gi, it, w, us);
+        g2p_one_brick<PB, ORDER, T, false, false, NT, R>(gi, it, w, us);
     });
 }
 
@@ -382,8 +382,17 @@ int batch_g2p(pic_context* ctx, pic_batch* b, const pic_pfield* x, const pic_pfi
     if (const int rc = batch_walk_items(ctx, b)) return rc;
     {
         TimedLaunch tl(ctx, "batch_g2p");
-        const unsigned nbk = persistent_blocks(ctx, k_batch_g2p_brick<ORDER, T>, BT, 0, b->nb);
-        k_batch_g2p_brick<ORDER, T><<<nbk, BT, 0, ctx->stream>>>(g, b->inst, b->shift);
+        // 256 work-items x 3 particles unless the context asks for 128 x 4
+        // (PIC_OPT_G2P_GROUP)
+        if (ctx->g2p_group == 128) {
+            const unsigned nbk =
+                persistent_blocks(ctx, k_batch_g2p_brick<ORDER, T, 128, 4>, 128, 0, b->nb);
+            k_batch_g2p_brick<ORDER, T, 128, 4><<<nbk, 128, 0, ctx->stream>>>(g, b->inst, b->shift);
+        } else {
+            const unsigned nbk =
+                persistent_blocks(ctx, k_batch_g2p_brick<ORDER, T, BT, 3>, BT, 0, b->nb);
+            k_batch_g2p_brick<ORDER, T, BT, 3><<<nbk, BT, 0, ctx->stream>>>(g, b->inst, b->shift);
+        }
         if (launch_status()) return PIC_ERR_HIP;
     }
     TimedLaunch tl(ctx, "batch_g2p_stale");
