@@ -92,18 +92,24 @@ def _check(gu, wl, insts, C, order, f32):
         assert gu.rel_inf(got[:, 1:], rc[:, 1:]) < tol_b, w["n"]
 
 
+@pytest.mark.parametrize("group", [0, 128])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-def test_batch_step_vs_oracle(pa, dtype):
+def test_batch_step_vs_oracle(pa, dtype, group):
     """32 instances, quadratic splines, one step from shuffled storage (the
-    sort's full path)."""
+    sort's full path), with the g2p workgroups of the default shape and of
+    128 work-items (PIC_OPT_G2P_GROUP)."""
     import gpu_util as gu
     f32 = dtype == "f32"
     tdt = torch.float32 if f32 else torch.float64
     wl = _workload(32, 8, 24, 0x5EED0003)
     insts, X, M, C = _build(pa, wl, 2, tdt, f32)
     batch = pa.Batch(gu.ctx(), 2, tdt, insts)
-    batch.step(gu.ctx(), X, M, C)
-    gu.ctx().synchronize()
+    gu.ctx().set_g2p_group(group)
+    try:
+        batch.step(gu.ctx(), X, M, C)
+        gu.ctx().synchronize()
+    finally:
+        gu.ctx().set_g2p_group(0)
     _check(gu, wl, insts, C, 2, f32)
     batch.close()
 
