@@ -497,7 +497,7 @@ int pic_batch_apic_step(pic_context* ctx, pic_batch* batch, const pic_pfield* x,
  * workgroup of the collocated APIC g2p brick kernels without the F update
  * (pic_apic_g2p, pic_apic_g2p_normalized, pic_apic_step and the batches'
  * g2p), 128 (four particles per work-item at a time) or 256 (three); 0
- * restores the default (128 for FP32, 256 for FP64 and for batches).
+ * restores the default (256).
  * Results do not depend on it. */
 enum { PIC_OPT_G2P_GROUP = 1 };
 int pic_set_option(pic_context* ctx, int option, int value);

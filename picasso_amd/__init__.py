@@ -125,7 +125,7 @@ class Context:
     def set_g2p_group(self, work_items):
         """Work-items per workgroup of the collocated APIC g2p brick kernels
         (pic_set_option PIC_OPT_G2P_GROUP): 128, 256, or 0 for the default
-        of each precision.  Results do not depend on it."""
+        (256).  Results do not depend on it."""
         check(lib.pic_set_option(self.handle, _abi.OPT_G2P_GROUP, int(work_items)),
               "pic_set_option")
 

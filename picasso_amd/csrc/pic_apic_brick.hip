@@ -82,8 +82,8 @@ __global__ __launch_bounds__(NT) void k_apic_g2p_brick(BrickArgs<T> g)
     constexpr int FN = BB + ORDER;
     __shared__ T us[(STAG ? 1 : 3) * FN * FN * FN];
     __shared__ BrickItem q[2];
-    walk_bricks<false>(g, q, [&](const BrickItem& it, BrickWalk<false, BrickArgs<T>>& w) {
-        g2p_one_brick<BB, ORDER, T, GF, STAG, NT, R>(g, it, w, us);
+    walk_bricks<true>(g, q, [&](const BrickItem& it, BrickWalk<true, BrickArgs<T>>& w) {
+        g2p_one_brick<BB, ORDER, T, GF, STAG, NT, R, true>(g, it, w, us);
     });
 }
 
@@ -243,7 +243,8 @@ static int g2p_brick_run(pic_context* ctx, int order, int entity, const pic_mesh
 // windows, the walk counters and both stale counts cleared by its key pass
 // and the walk items listed by its scan, then the p2g and normalised g2p
 // brick kernels and their stale lists.  Scratch: the sort's layout, then
-// walk counters | p2g stale list (1 + np words) | g2p stale list | items.
+// p2g walk counters | g2p walk counters | p2g stale list (1 + np words) |
+// g2p stale list | items.
 template <typename T>
 static int step_transfers(pic_context* ctx, int order, int entity, int dtype,
                           const pic_mesh& mesh, int64_t np, const pic_pfield& x,
@@ -268,6 +269,7 @@ static int step_transfers(pic_context* ctx, int order, int entity, int dtype,
                                      bins.perm, stale_p2g, false))
         return rc;
     BrickArgs<T> h = g;
+    h.walk = ctr + size_t(WALK_LANES) * WALK_PAD;
     h.stale = stale_g2p;
     h.a = make_args<T>(ctx, entity, mesh, np, &x, nullptr, &c, nullptr, gmu, bins.perm);
     h.a.gnm = make_gview<T>(gm);
@@ -295,13 +297,14 @@ int apic_step_bricks(pic_context* ctx, int order, int entity, int dtype, const p
     const size_t sort_bytes = (L.bytes + 255) & ~size_t(255);
     const size_t ctr_words = size_t(WALK_LANES) * WALK_PAD;
     const size_t stale_words = 1 + size_t(np);
+    // p2g's walk counters, g2p's, then the two stale lists
     const size_t items_at =
-        (sort_bytes + (ctr_words + 2 * stale_words) * sizeof(uint32_t) + 15) & ~size_t(15);
+        (sort_bytes + (2 * ctr_words + 2 * stale_words) * sizeof(uint32_t) + 15) & ~size_t(15);
     char* base = static_cast<char*>(scratch(ctx, items_at + size_t(nb) * sizeof(BrickItem)));
     if (!base) return PIC_ERR_ALLOC;
     uint32_t* sc = reinterpret_cast<uint32_t*>(base);
     uint32_t* ctr = reinterpret_cast<uint32_t*>(base + sort_bytes);
-    uint32_t* stale_p2g = ctr + ctr_words;
+    uint32_t* stale_p2g = ctr + 2 * ctr_words;
     uint32_t* stale_g2p = stale_p2g + stale_words;
     BrickItem* items = reinterpret_cast<BrickItem*>(base + items_at);
 
@@ -319,7 +322,7 @@ int apic_step_bricks(pic_context* ctx, int order, int entity, int dtype, const p
             if (const int rc = zero_grid(ctx, dtype, *grids[k], gcomp[k])) return rc;
         }
     }
-    z.add(ctr, int64_t(ctr_words * sizeof(uint32_t)));
+    z.add(ctr, int64_t(2 * ctr_words * sizeof(uint32_t)));
     z.add(stale_p2g, sizeof(uint32_t));
     z.add(stale_g2p, sizeof(uint32_t));
 

@@ -37,17 +37,16 @@ namespace pic {
 constexpr int BT = 256;  // work-items per g2p brick workgroup (batches, FP64, F update)
 
 // Work-items per workgroup and particles per work-item at a time of a
-// single-instance g2p brick kernel.  FP32 collocated g2p without the F
-// update runs 128 work-items with four particles each: on the 256^3 sphere
-// g2p 0.287 -> 0.275 ms, the step 0.879 -> 0.871 ms.  The others are faster
-// at 256 x 3: FP64 (1.753 against 1.769 ms per step), the staggered
-// faces (MAC step 4.26 against 4.46 ms) and the batched FP32 g2p over 512
-// instances (6.39 against 6.57 ms).
+// single-instance g2p brick kernel: 256 x 3 for every variant.  Under the
+// dynamic brick walk it beats 128 x 4 on the 256^3 sphere in FP32 (g2p
+// 0.253 against 0.262 ms inside the one-call step) and FP64 (0.437 against
+// 0.461 ms); 128 x 4 was the faster FP32 shape only while a static walk
+// left a tail of slow workgroups.  The collocated kernel without the F
+// update also runs 128 x 4 on request (PIC_OPT_G2P_GROUP).
 template <typename T, bool GF, bool STAG>
 struct G2PShape {
-    static constexpr bool SMALL = sizeof(T) == 4 && !GF && !STAG;
-    static constexpr int NT = SMALL ? 128 : BT;
-    static constexpr int R = SMALL ? 4 : 3;
+    static constexpr int NT = BT;
+    static constexpr int R = 3;
 };
 constexpr int PB = 4;    // p2g brick edge in bins
 
@@ -524,9 +523,10 @@ __device__ __forceinline__ void p2g_one_brick(const BrickArgs<T>& g, const Brick
     }
 }
 
-template <int BB, int ORDER, typename T, bool GF, bool STAG, int NT = BT, int R = 3>
+template <int BB, int ORDER, typename T, bool GF, bool STAG, int NT = BT, int R = 3,
+          bool DYN = false>
 __device__ __forceinline__ void g2p_one_brick(const BrickArgs<T>& g, const BrickItem& it,
-                                              BrickWalk<false, BrickArgs<T>>& w, T* us)
+                                              BrickWalk<DYN, BrickArgs<T>>& w, T* us)
 {
     constexpr int FN = BB + ORDER;
     constexpr int FV = FN * FN * FN;

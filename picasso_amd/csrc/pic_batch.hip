@@ -178,11 +178,11 @@ __global__ __launch_bounds__(NT) void k_batch_g2p_brick(BrickArgs<T> g,
     constexpr int FN = PB + ORDER;
     __shared__ T us[3 * FN * FN * FN];
     __shared__ BrickItem q[2];
-    walk_bricks<false>(g, q, [&](const BrickItem& it, BrickWalk<false, BrickArgs<T>>& w) {
+    walk_bricks<true>(g, q, [&](const BrickItem& it, BrickWalk<true, BrickArgs<T>>& w) {
         const uint32_t k = __builtin_amdgcn_readfirstlane(it.b) >> shift;
         BrickArgs<T> gi = g;
         instance_args(inst[k], gi, true);
-        g2p_one_brick<PB, ORDER, T, false, false, NT, R>(gi, it, w, us);
+        g2p_one_brick<PB, ORDER, T, false, false, NT, R, true>(gi, it, w, us);
     });
 }
 

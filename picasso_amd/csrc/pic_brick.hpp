@@ -125,8 +125,9 @@ struct BrickWalk {
 };
 
 // Drive body(item, walk) over the walk.  Without DYN the walk is a plain
-// grid stride (workgroup b takes b, b + grid, ...), which suits kernels
-// without atomics whose bricks cost about the same.  With DYN, when the grid
+// grid stride (workgroup b takes b, b + grid, ...; in the XCD-contiguous
+// order below) that needs no counters: the colour passes use it, one launch
+// per colour with no clear in between.  With DYN, when the grid
 // is a multiple of WALK_LANES, workgroup b (b = 8m + r) starts at walk index
 // 8m + lane with lane = (r + m) mod 8 and then draws from counter `lane`;
 // so each lane's workgroups sit on every XCD (blockIdx modulo 8 picks the

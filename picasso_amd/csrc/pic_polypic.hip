@@ -586,7 +586,7 @@ __device__ __forceinline__ void polypic_p2g_one_brick(const PolyBrickArgs<T>& g,
 template <int ORDER, typename T, bool STAG>
 __device__ __forceinline__ void polypic_g2p_one_brick(const PolyBrickArgs<T>& g,
                                                       const BrickItem& it,
-                                                      BrickWalk<false, PolyBrickArgs<T>>& w,
+                                                      BrickWalk<true, PolyBrickArgs<T>>& w,
                                                       T* us)
 {
     constexpr int PB = 4, NT = 256;
@@ -699,7 +699,7 @@ __global__ __launch_bounds__(256) void k_polypic_g2p_brick(PolyBrickArgs<T> g)
     constexpr int FN = 4 + ORDER;
     __shared__ T us[(STAG ? 1 : 3) * FN * FN * FN];
     __shared__ BrickItem q[2];
-    walk_bricks<false>(g, q, [&](const BrickItem& it, BrickWalk<false, PolyBrickArgs<T>>& w) {
+    walk_bricks<true>(g, q, [&](const BrickItem& it, BrickWalk<true, PolyBrickArgs<T>>& w) {
         polypic_g2p_one_brick<ORDER, T, STAG>(g, it, w, us);
     });
 }

@@ -34,7 +34,7 @@ class StepConfig:
     resort: int = 1
     permute: Optional[int] = None
     g2p_brick: int = 4
-    g2p_group: int = 0  # 0: the library's default for the precision
+    g2p_group: int = 0  # 0: the library's default (256)
 
     def as_dict(self):
         return {"resort": self.resort, "permute": self.permute, "g2p_brick": self.g2p_brick,
